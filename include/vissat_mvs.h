@@ -1,0 +1,223 @@
+/*
+ * vissat_mvs.h -- C ABI of libvissat_mvs.so, the MI355X (gfx950) PatchMatch
+ * MVS + 2.5D DSM fusion library.
+ *
+ * The reference crosses its native boundary as a CLI subprocess:
+ *   colmap patch_match_stereo --workspace_path <mvs_dir> --PatchMatchStereo.<flag> <v> ...
+ * (reference colmap_mvs_commands.py:40-54, spawned at lib/run_cmd.py:44), and
+ * fuses depth maps in Python (aggregate_2p5d.py:45-89,
+ * aggregate_2p5d_util.py:45-106, lib/proj_to_grid.py:41-81).  This header is
+ * the in-process replacement: plain C types, caller-owned host buffers copied
+ * in and out, device state owned by the context.
+ *
+ * Conventions: every call returns 0 (VSM_OK) or a negative VSM_E* code; no
+ * exception crosses the ABI; vsm_last_error() gives the message of the last
+ * failure on that context.  A context is used by one host thread at a time
+ * (callers serialise), except vsm_submit/vsm_poll which the serving loop
+ * drives from its own thread.
+ */
+#ifndef VISSAT_MVS_H
+#define VISSAT_MVS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VSM_OK 0
+#define VSM_EINVAL (-22)   /* bad argument (size, id, pointer, parameter) */
+#define VSM_ENOMEM (-12)   /* allocation failed or HBM budget exceeded */
+#define VSM_EDEVICE (-5)   /* HIP runtime / kernel failure */
+#define VSM_ENOENT (-2)    /* unknown image id / ticket */
+#define VSM_EAGAIN (-11)   /* serving: not admitted yet (over budget, queued) */
+
+#define VSM_MAX_SRC 32     /* sources per reference view */
+#define VSM_MAX_RADIUS 8   /* window_radius upper bound */
+
+typedef struct vsm_ctx vsm_ctx;
+
+/* PatchMatchStereo options.  Defaults (vsm_default_params) are the values of
+ * the reference's photometric call, colmap_mvs_commands.py:40-48:
+ * window_radius 3 (caller stereo_pipeline.py:444), min_triangulation_angle
+ * 5.0, filter 0, geom_consistency 0, num_samples 15, num_iterations 12; the
+ * consistency call (colmap_mvs_commands.py:60-75) sets geom_consistency 1,
+ * geom_consistency_regularizer 0.3, geom_consistency_max_cost 3, filter 1,
+ * filter_min_triangulation_angle 4.99, filter_min_ncc 0.1,
+ * filter_geom_consistency_max_cost 1, filter_min_num_consistent 2,
+ * num_iterations 1. */
+typedef struct {
+    int32_t window_radius;
+    int32_t window_step;              /* tap stride inside the window (1 or 2) */
+    float sigma_spatial;              /* <= 0: window_radius */
+    float sigma_color;                /* on [0,1] intensities */
+    float ncc_sigma;
+    int32_t num_samples;              /* view-selection Monte-Carlo draws, <= 16 */
+    int32_t num_iterations;
+    float min_triangulation_angle;    /* degrees */
+    float incident_angle_sigma;       /* radians */
+    int32_t geom_consistency;
+    float geom_consistency_regularizer;
+    float geom_consistency_max_cost;  /* pixels */
+    int32_t filter;
+    float filter_min_ncc;
+    float filter_min_triangulation_angle;   /* degrees */
+    float filter_geom_consistency_max_cost; /* pixels */
+    int32_t filter_min_num_consistent;
+    float depth_min, depth_max;       /* > 0 overrides the per-request range */
+    float perturbation;               /* depth perturbation, fraction of range */
+    float normal_perturbation;        /* radians */
+    float max_normal_angle;           /* degrees, cone of random normals */
+    float perturb_decay;              /* per-iteration shrink of perturbations */
+    int32_t n_far;                    /* propagation candidates per direction */
+    uint64_t seed;
+} vsm_params;
+
+/* One reference view to solve. */
+typedef struct {
+    int32_t ref_id;                   /* image id given to vsm_add_image */
+    int32_t n_src;
+    const int32_t *src_ids;           /* n_src image ids */
+    float depth_min, depth_max;       /* depth_ranges.txt (reparam_depth.py:187-189) */
+    /* Optional initial hypotheses for the geometric pass when the context
+     * holds no photometric result for ref_id (resume from files,
+     * use_exist_photom, colmap_mvs_commands.py:64): H*W depth and H*W*3
+     * camera-frame normals, as read from the workspace. NULL otherwise. */
+    const float *init_depth;
+    const float *init_normal;
+} vsm_view_req;
+
+/* Caller-owned outputs; any pointer may be NULL (result stays on device). */
+typedef struct {
+    float *depth;                     /* H*W, row-major; 0 = no depth */
+    float *normal;                    /* H*W*3, camera frame, facing the camera */
+    float *cost;                      /* H*W aggregated cost */
+    uint32_t *consistency;            /* H*W bitmask of consistent sources (filter) */
+} vsm_view_out;
+
+/* DSM grid in the frame of the back-projected points (ENU metres):
+ * row = floor((ul_n - N)/e_res), col = floor((E - ul_e)/n_res), keeping the
+ * reference's pairing of x/y resolutions (lib/proj_to_grid.py:42-43, called
+ * with (e_resolution, n_resolution) at produce_dsm.py:56); the two are equal
+ * in every configuration the reference uses (produce_dsm.py:41-42). */
+typedef struct {
+    double ul_e, ul_n;
+    double e_res, n_res;
+    int32_t e_size, n_size;
+} vsm_dsm_grid;
+
+/* Kernel timing and work counters accumulated since the last reset. */
+typedef struct {
+    double halfstep_ms;               /* device time of half-step kernels */
+    int64_t halfstep_launches;
+    int64_t ncc_evals;                /* (pixel, hypothesis, source) NCC evaluations */
+    int64_t geo_evals;
+    int64_t taps;                     /* ncc_evals * window taps */
+    double fuse_ms;
+    int64_t fuse_launches;
+    int64_t fuse_pixels;
+    uint64_t hbm_bytes_in_use;
+    uint64_t hbm_bytes_peak;
+} vsm_stats;
+
+void vsm_default_params(vsm_params *p);
+
+int vsm_create(int device, uint64_t hbm_budget_bytes, vsm_ctx **out);
+int vsm_destroy(vsm_ctx *ctx);
+const char *vsm_last_error(vsm_ctx *ctx);
+int vsm_set_params(vsm_ctx *ctx, const vsm_params *p);
+int vsm_get_params(vsm_ctx *ctx, vsm_params *p);
+
+/* Upload one grayscale image and its FP64 4x4 P4 (row-major), the
+ * plane-plus-parallax projection [K[R|t]; depth_min*(0,0,1,-z0)] of
+ * reparam_depth.py:136-141.  Replaces the binary reading images/ and
+ * sparse/ from the workspace.  gray may be NULL to register geometry only
+ * (fusion of depth maps read from files); such an image cannot be solved. */
+int vsm_add_image(vsm_ctx *ctx, int32_t image_id, const uint8_t *gray, int32_t w, int32_t h,
+                  int32_t row_stride, const double P4[16]);
+int vsm_remove_image(vsm_ctx *ctx, int32_t image_id);
+
+/* Provide a depth map for image_id from outside: kind 0 photometric (read
+ * back from depth_maps/<img>.photometric.bin for the consistency pass,
+ * use_exist_photom, colmap_mvs_commands.py:64), kind 1 geometric (for
+ * fusion of maps read from files, aggregate_2p5d_util.py:75). */
+int vsm_set_depth_map(vsm_ctx *ctx, int32_t image_id, int32_t kind, const float *depth, int32_t w,
+                      int32_t h);
+
+/* Solve n reference views in one batched launch sequence and block until
+ * done.  pass 0 = photometric (run_photometric_mvs,
+ * colmap_mvs_commands.py:39-55), pass 1 = geometric consistency + filter
+ * (run_consistency_check, colmap_mvs_commands.py:58-82).  With
+ * use_graph != 0 the half-step sequence is captured once in a hipGraph and
+ * replayed. */
+int vsm_solve(vsm_ctx *ctx, const vsm_view_req *reqs, int32_t n, int32_t pass,
+              vsm_view_out *outs, int32_t use_graph);
+
+/* Serving: non-blocking admission.  The request is copied; it is admitted
+ * when its HBM footprint fits the budget (else it waits in FIFO order), joins
+ * the running batch at the next half-step boundary, and runs the photometric
+ * pass (num_iterations) followed by the filter when params.filter is set
+ * (photometric consistency only: NCC and triangulation angle).  A request's
+ * result does not depend on which other requests share its launches.
+ * vsm_poll copies the result into out when done (*done = 1), else *done = 0.
+ * vsm_serve_step admits what fits and advances every admitted request by one
+ * half-step (one batched launch). */
+int vsm_submit(vsm_ctx *ctx, const vsm_view_req *req, uint64_t *ticket);
+int vsm_serve_step(vsm_ctx *ctx, int32_t *n_active, int32_t *n_queued);
+int vsm_poll(vsm_ctx *ctx, uint64_t ticket, vsm_view_out *out, int32_t *done);
+
+/* Per-pixel cost of fixed hypotheses against every source: out[s*H*W + p].
+ * depth H*W, normal H*W*3 in the (col,row,d) frame used by the solver.
+ * pass 1 adds the geometric term.  Test entry point for the
+ * "cost under fixed hypotheses" parity check. */
+int vsm_eval_cost(vsm_ctx *ctx, const vsm_view_req *req, int32_t pass, const float *depth,
+                  const float *normal_uvd, float *out);
+
+/* One red-black half-step on host-provided state (in/out): depth H*W,
+ * normal_uvd H*W*3, cost H*W, prob n_src*H*W.  Test entry point for
+ * lockstep parity against the oracle. */
+int vsm_debug_halfstep(vsm_ctx *ctx, const vsm_view_req *req, int32_t pass, int32_t iter,
+                       int32_t color, float *depth, float *normal_uvd, float *cost, float *prob);
+/* Random initialisation of the state (depth, normal_uvd, cost, prob). */
+int vsm_debug_init(vsm_ctx *ctx, const vsm_view_req *req, int32_t pass, float *depth,
+                   float *normal_uvd, float *cost, float *prob);
+/* Filter on host-provided state; writes depth/normal_uvd in place and the mask. */
+int vsm_debug_filter(vsm_ctx *ctx, const vsm_view_req *req, int32_t pass, float *depth,
+                     float *normal_uvd, uint32_t *mask);
+
+/* 2.5D fusion of the depth maps of n images held by the context into the DSM
+ * grid: per view back-projection through inv(P4) (aggregate_2p5d_util.py:86-91),
+ * per-cell nanmax + 3x3 hole fill (lib/proj_to_grid.py:41-81), 3x3 median
+ * (produce_dsm.py:58); across views the <= (min_measurements-1) rejection,
+ * MAD outlier removal, mean and 3x3 median (aggregate_2p5d.py:66-81).
+ * depth_kind: 0 photometric, 1 geometric (aggregate_2p5d.py:48).
+ * inv_proj_mats: optional n*16 FP64 row-major inverse projections as read
+ * from inv_proj_mats.txt (aggregate_2p5d_util.py:55-61); NULL uses the
+ * context's own inverse of each P4.
+ * dsm_out: n_size*e_size float (NaN = empty); per_view_out: optional
+ * n*n_size*e_size per-view DSMs. */
+int vsm_fuse_dsm(vsm_ctx *ctx, const vsm_dsm_grid *grid, const int32_t *image_ids, int32_t n,
+                 int32_t depth_kind, int32_t min_measurements, const double *inv_proj_mats,
+                 float *dsm_out, float *per_view_out);
+
+/* Stand-alone rasteriser with the semantics of proj_to_grid
+ * (lib/proj_to_grid.py:41-81): points n x 3 FP64 (E, N, h) -> n_size x
+ * e_size FP64 grid with per-cell nanmax and the 3x3 hole fill. */
+int vsm_proj_to_grid(vsm_ctx *ctx, const double *points, int64_t n, const vsm_dsm_grid *grid,
+                     double *dsm_out);
+
+/* 3x3 median with replicated border on an n_size x e_size float raster
+ * (cv2.medianBlur(dsm, 3) at produce_dsm.py:58 and aggregate_2p5d.py:81);
+ * NaN excluded from each window, all-NaN -> NaN. */
+int vsm_median_blur3(vsm_ctx *ctx, const float *in, int32_t e_size, int32_t n_size, float *out);
+
+int vsm_get_stats(vsm_ctx *ctx, vsm_stats *out);
+int vsm_reset_stats(vsm_ctx *ctx);
+/* Enable (1) or disable (0) per-launch event timing of half-step kernels. */
+int vsm_set_timing(vsm_ctx *ctx, int32_t enabled);
+int vsm_synchronize(vsm_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VISSAT_MVS_H */

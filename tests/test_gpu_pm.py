@@ -1,0 +1,247 @@
+"""HIP PatchMatch against the FP64 oracle (oracle/pm_oracle.c) on the GPU.
+
+* cost of fixed hypotheses: |GPU - oracle| <= 1e-4 per (pixel, source);
+* random initialisation: identical Philox streams -> hypotheses within FP32
+  rounding of the oracle's;
+* lockstep half-steps: the oracle runs one half-step from the GPU's own
+  state; the winning hypothesis must agree wherever the oracle's candidate
+  costs are separated by more than 1e-4 (the documented tie tolerance), and
+  every final cost must agree within 1e-4 (plus a tiny allowance for the
+  rare Monte-Carlo draw that lands within FP32 rounding of a CDF boundary);
+* full solves: depth accuracy against the synthetic ground truth, and GPU
+  vs oracle agreement statistics.
+"""
+import numpy as np
+import pytest
+
+import sites
+
+pytestmark = pytest.mark.gpu
+
+COST_TOL = 1e-4
+
+
+def _ctx(**params):
+    from vissatsatellitestereo_amd.mvs import MvsContext
+    return MvsContext(device=0, **params)
+
+
+def _oracle(site, ref, srcs, params, src_depths=None):
+    import pm_oracle as po
+    return po.OracleView(site["imgs"][ref], [site["imgs"][s] for s in srcs], site["P4"][ref],
+                         [site["P4"][s] for s in srcs], site["dr"][ref], view_id=_id(site, ref),
+                         src_depths=src_depths, params=params)
+
+
+def _id(site, name):
+    return site["names"].index(name) + 1
+
+
+def _params(**kw):
+    p = dict(window_radius=5, window_step=1, num_iterations=5, num_samples=15)
+    p.update(kw)
+    return p
+
+
+def _gpu_params(p):
+    keys = ("window_radius", "window_step", "num_samples", "num_iterations", "sigma_color",
+            "ncc_sigma", "min_triangulation_angle", "incident_angle_sigma",
+            "geom_consistency_regularizer", "geom_consistency_max_cost", "filter_min_ncc",
+            "filter_min_triangulation_angle", "filter_geom_consistency_max_cost",
+            "filter_min_num_consistent", "perturbation", "normal_perturbation",
+            "max_normal_angle", "perturb_decay", "n_far", "seed")
+    import pm_oracle as po
+    full = po.default_params(**p)
+    return {k: full[k] for k in keys if k in full}
+
+
+def _setup(site, params, ref_i=0, n_src=None):
+    names = site["names"]
+    ref = names[ref_i]
+    srcs = [n for n in names if n != ref][:n_src]
+    ctx = _ctx(**_gpu_params(params))
+    for n in names:
+        ctx.add_image(_id(site, n), site["imgs"][n], site["P4"][n])
+    return ctx, ref, srcs
+
+
+@pytest.fixture(scope="module")
+def c1(site_dir):
+    from vissatsatellitestereo_amd import synth
+    return sites.build("c1", synth.spec_c1(), site_dir)
+
+
+@pytest.fixture(scope="module")
+def tiny(site_dir):
+    return sites.build("tiny", sites.tiny_spec(), site_dir)
+
+
+def _random_hyp(site, ref, seed=7):
+    h, w = site["imgs"][ref].shape
+    rng = np.random.default_rng(seed)
+    dmin, dmax = site["dr"][ref]
+    d = rng.uniform(dmin, dmax, (h, w))
+    n = rng.normal(size=(h, w, 3)) * [0.3, 0.3, 1.0]
+    n[..., 2] = np.abs(n[..., 2]) + 0.3
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    return d.astype(np.float32), n.astype(np.float32)
+
+
+def test_cost_fixed_hypotheses_c1(c1):
+    p = _params()
+    ctx, ref, srcs = _setup(c1, p)
+    d, n = _random_hyp(c1, ref)
+    # near-true depths too, so most windows are well-matched (NCC ~ 1)
+    gt = sites.gt_depth(c1, ref).astype(np.float32)
+    d[::2] = gt[::2]
+    n[::2] = [0, 0, 1]
+    g = ctx.eval_cost(_id(c1, ref), [_id(c1, s) for s in srcs], c1["dr"][ref], d, n, 0)
+    o = _oracle(c1, ref, srcs, p).cost_map(d.astype(np.float64), n.astype(np.float64), 0)
+    err = np.abs(g - o)
+    print("cost |gpu-oracle| max", err.max(), "mean", err.mean(), "frac matched(<0.2)",
+          (o < 0.2).mean())
+    assert (o < 0.3).mean() > 0.2          # the test covers well-matched windows
+    assert err.max() <= COST_TOL, np.unravel_index(err.argmax(), err.shape)
+
+
+def test_cost_fixed_hypotheses_multisource_stride2(tiny):
+    p = _params(window_radius=4, window_step=2)
+    ctx, ref, srcs = _setup(tiny, p, ref_i=1)
+    d, n = _random_hyp(tiny, ref, seed=3)
+    g = ctx.eval_cost(_id(tiny, ref), [_id(tiny, s) for s in srcs], tiny["dr"][ref], d, n, 0)
+    o = _oracle(tiny, ref, srcs, p).cost_map(d.astype(np.float64), n.astype(np.float64), 0)
+    err = np.abs(g - o)
+    print("cost stride2 max", err.max())
+    assert err.max() <= COST_TOL
+
+
+def test_init_matches_oracle(c1):
+    p = _params()
+    ctx, ref, srcs = _setup(c1, p)
+    st = ctx.debug_init(_id(c1, ref), [_id(c1, s) for s in srcs], c1["dr"][ref])
+    ov = _oracle(c1, ref, srcs, p)
+    so = ov.new_state()
+    ov.init(so, 0)
+    assert np.allclose(st["depth"], so["depth"], rtol=1e-6, atol=1e-5)
+    nerr = np.abs(st["normal"] - so["normal"])
+    print("init normal max err", nerr.max())
+    assert nerr.max() <= 1e-5
+    assert np.all(st["cost"] == 2.0) and np.all(st["prob"] == 0.5)
+
+
+def _lockstep(site, p, ref_i, n_src, pass_, steps, src_depths_gpu=None):
+    """Run `steps` half-steps on the GPU; after each, run the oracle from the
+    GPU's previous state and compare."""
+    import pm_oracle as po
+    ctx, ref, srcs = _setup(site, p, ref_i, n_src)
+    rid, sids = _id(site, ref), [_id(site, s) for s in srcs]
+    ov = _oracle(site, ref, srcs, p)
+    if pass_ == 1:
+        for s in srcs:
+            ctx.set_depth_map(_id(site, s), src_depths_gpu[s], kind=0)
+        ov.set_source_depths([src_depths_gpu[s] for s in srcs])
+    st = ctx.debug_init(rid, sids, site["dr"][ref], pass_)
+    H, W = site["imgs"][ref].shape
+    report = []
+    for k in range(steps):
+        it, color = k // 2, k % 2
+        nxt = ctx.debug_halfstep(rid, sids, site["dr"][ref], st, pass_, it, color)
+        so = {key: st[key].astype(np.float64).copy() for key in ("depth", "normal", "cost", "prob")}
+        win, cand = ov.halfstep(so, pass_, it, color, diagnostics=True)
+        yy, xx = np.mgrid[0:H, 0:W]
+        act = ((xx + yy) % 2 == color).reshape(-1)
+        dg, do = nxt["depth"][act], so["depth"][act]
+        ng = nxt["normal"].reshape(-1, 3)[act]
+        no = so["normal"].reshape(-1, 3)[act]
+        same = (np.abs(dg - do) <= 1e-3 * (1 + np.abs(do))) & (np.abs(ng - no).max(1) <= 1e-3)
+        c = cand[act]
+        srt = np.sort(c, axis=1)
+        gap = srt[:, 1] - srt[:, 0]
+        decided = gap > COST_TOL
+        cerr = np.abs(nxt["cost"][act] - so["cost"][act])
+        perr = np.abs(nxt["prob"].reshape(-1, H * W)[:, act] - so["prob"].reshape(-1, H * W)[:, act])
+        r = dict(step=k, mismatch=float((~same).mean()), decided_mismatch=float((~same & decided).mean()),
+                 cost_err_max=float(cerr.max()), cost_err_p999=float(np.quantile(cerr, 0.999)),
+                 frac_cost_err=float((cerr > COST_TOL).mean()), prob_err_max=float(perr.max()))
+        print(r)
+        report.append(r)
+        st = nxt
+    return report, st
+
+
+def test_lockstep_photometric_c1(c1):
+    rep, _ = _lockstep(c1, _params(), 0, None, 0, 6)
+    for r in rep:
+        assert r["decided_mismatch"] <= 2e-4, r
+        assert r["frac_cost_err"] <= 2e-4, r
+        assert r["cost_err_p999"] <= COST_TOL, r
+
+
+def test_lockstep_photometric_multisource(tiny):
+    rep, _ = _lockstep(tiny, _params(window_radius=4), 0, 3, 0, 6)
+    for r in rep:
+        assert r["decided_mismatch"] <= 1e-3, r
+        assert r["frac_cost_err"] <= 1e-3, r
+
+
+def test_full_solve_accuracy_and_oracle_agreement(c1):
+    p = _params()
+    ctx, ref, srcs = _setup(c1, p)
+    res = ctx.solve([dict(ref=_id(c1, ref), srcs=[_id(c1, s) for s in srcs],
+                          depth_range=c1["dr"][ref])], pass_=0)[0]
+    gt = sites.gt_depth(c1, ref)
+    ov = _oracle(c1, ref, srcs, p)
+    so = ov.solve(0, p["num_iterations"])
+    do = so["depth"].reshape(gt.shape)
+    m = np.zeros(gt.shape, bool)
+    m[8:-8, 8:-8] = True
+    eg = np.abs(res["depth"] - gt)[m]
+    eo = np.abs(do - gt)[m]
+    print("GPU median |dd|", np.median(eg), "<1m", (eg < 1).mean(), " oracle", np.median(eo), (eo < 1).mean())
+    assert np.median(eg) <= np.median(eo) + 0.05
+    assert abs((eg < 1).mean() - (eo < 1).mean()) < 0.03
+    # the same hypothesis at most pixels (independent FP32/FP64 runs)
+    agree = np.abs(res["depth"] - do)[m] < 0.05
+    print("GPU/oracle depth agreement", agree.mean())
+    assert agree.mean() > 0.6
+    assert np.all(np.isfinite(res["depth"])) and np.all(np.isfinite(res["normal"]))
+    nz = res["normal"][m]
+    assert np.allclose(np.linalg.norm(nz, axis=-1), 1, atol=1e-4)
+    assert np.all(nz[:, 2] <= 0)
+
+
+def test_batched_equals_single(tiny):
+    p = _params(window_radius=4, num_iterations=3)
+    ctx, _, _ = _setup(tiny, p)
+    views = []
+    for r in tiny["names"]:
+        srcs = [n for n in tiny["names"] if n != r]
+        views.append(dict(ref=_id(tiny, r), srcs=[_id(tiny, s) for s in srcs], depth_range=tiny["dr"][r]))
+    batch = ctx.solve(views, 0)
+    graph = ctx.solve(views, 0, use_graph=True)
+    for v, b, g in zip(views, batch, graph):
+        one = ctx.solve([v], 0)[0]
+        assert np.array_equal(one["depth"], b["depth"])
+        assert np.array_equal(one["normal"], b["normal"])
+        assert np.array_equal(g["depth"], b["depth"])
+
+
+def test_geometric_pass_and_filter(tiny):
+    p = _params(window_radius=4, num_iterations=3)
+    ctx, _, _ = _setup(tiny, p)
+    names = tiny["names"]
+    views = [dict(ref=_id(tiny, r), srcs=[_id(tiny, s) for s in names if s != r],
+                  depth_range=tiny["dr"][r]) for r in names]
+    photo = ctx.solve(views, 0)
+    ctx.set_params(geom_consistency=1, filter=1)
+    geo = ctx.solve(views, 1)
+    for n, ph, ge in zip(names, photo, geo):
+        gt = sites.gt_depth(tiny, n)
+        valid = ge["depth"] > 0
+        ep = np.abs(ph["depth"] - gt)
+        eg = np.abs(ge["depth"] - gt)[valid]
+        print(n, "photo med", np.median(ep), "geo(valid) med", np.median(eg), "kept", valid.mean())
+        assert valid.mean() > 0.3
+        assert np.median(eg) <= np.median(ep) + 0.05
+        pc = np.array([bin(int(m)).count("1") for m in ge["consistency"].ravel()]).reshape(valid.shape)
+        assert np.all(pc[~valid] < 2) and np.all(pc[valid] >= 2)

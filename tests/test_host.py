@@ -1,0 +1,194 @@
+"""Host-side tests (CPU only): reparam_depth, workspace files, array IO,
+source selection, the synthetic generator, and the C-ABI library exports."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import sites
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def tiny(site_dir):
+    return sites.build("tiny", sites.tiny_spec(), site_dir)
+
+
+def test_robust_depth_range_golden():
+    """reparam_depth.py:40-65 by hand: 50 values 1..50 -> sorted[int(0.02*50)=1]=2,
+    sorted[int(0.98*50)=49]=50 -> (2-10, 50+30); a single value v -> (v-10, v+30);
+    empty -> (-1e20, -1e20)."""
+    from vissatsatellitestereo_amd.reparam_depth import robust_depth_range
+    out = robust_depth_range({"a": list(range(50, 0, -1)), "b": [7.0], "c": []})
+    assert out["a"] == (2 - 10, 50 + 30)
+    assert out["b"] == (7.0 - 10, 7.0 + 30)
+    assert out["c"] == (-1e20, -1e20)
+
+
+def test_reparam_depth_matches_loop_oracle(tiny):
+    import reparam_oracle as ro
+    from vissatsatellitestereo_amd.reparam_depth import read_name_values
+    mvs = tiny["mvs"]
+    o = ro.reparam_depth(os.path.join(mvs, "sparse"))
+    raw = read_name_values(os.path.join(mvs, "raw_depth.txt"))
+    rep = read_name_values(os.path.join(mvs, "reparam_depth.txt"))
+    lr = read_name_values(os.path.join(mvs, "last_rows.txt"))
+    dr = read_name_values(os.path.join(mvs, "depth_ranges.txt"))
+    for n in tiny["names"]:
+        assert np.allclose(raw[n], o["raw_depth"][n], rtol=1e-12)
+        assert np.allclose(rep[n], o["reparam_depth"][n], rtol=1e-9, atol=1e-9)
+        assert np.allclose(lr[n], o["last_rows"][n], rtol=1e-12)
+        assert np.allclose(dr[n], o["depth_ranges"][n], rtol=1e-9, atol=1e-9)
+    plane = open(os.path.join(mvs, "reference_plane.txt")).read().split()
+    assert np.allclose([float(v) for v in plane], o["reference_plane"], rtol=1e-12)
+
+
+def test_reparam_depth_is_scaled_height(tiny):
+    """d = depth_min (z - z0) / z_cam: ground truth depth of the rendered
+    scene lies inside the common depth range."""
+    for n in tiny["names"]:
+        gt = sites.gt_depth(tiny, n)
+        lo, hi = tiny["dr"][n]
+        assert lo < np.percentile(gt, 1) and np.percentile(gt, 99) < hi
+
+
+def test_projection_round_trip(tiny):
+    """A ground-truth hit point projects through P4 to its pixel and depth."""
+    from vissatsatellitestereo_amd import synth
+    n = tiny["names"][0]
+    P4 = tiny["P4"][n]
+    hz = np.load(os.path.join(tiny["wd"], "synthetic_gt", n[:-4] + ".hit_z.npy"))
+    gt = sites.gt_depth(tiny, n)
+    inv = np.linalg.inv(P4)
+    for (r, c) in [(10, 20), (64, 80), (100, 150)]:
+        X = inv @ np.array([c, r, 1.0, gt[r, c]])
+        X = X[:3] / X[3]
+        assert abs(X[2] - hz[r, c]) < 1e-3
+        y = P4 @ np.append(X, 1.0)
+        assert abs(y[0] / y[2] - c) < 1e-6 and abs(y[1] / y[2] - r) < 1e-6
+    del synth
+
+
+def test_array_io_round_trip(tmp_path):
+    from vissatsatellitestereo_amd.colmap.read_dense import read_array, write_array
+    a = np.random.default_rng(0).normal(size=(5, 7)).astype(np.float32)
+    b = np.random.default_rng(1).normal(size=(5, 7, 3)).astype(np.float32)
+    write_array(a, tmp_path / "a.bin")
+    write_array(b, tmp_path / "b.bin")
+    assert np.array_equal(read_array(tmp_path / "a.bin"), a)
+    assert np.array_equal(read_array(tmp_path / "b.bin"), b)
+    raw = open(tmp_path / "b.bin", "rb").read()
+    assert raw.startswith(b"7&5&3&")
+    # column-major (w, h, c): w varies fastest, so the second float is
+    # pixel (row 0, col 1), channel 0, and float w*h is channel 1 of (0, 0)
+    vals = np.frombuffer(raw[6:], dtype="<f4")
+    assert vals[0] == b[0, 0, 0] and vals[1] == b[0, 1, 0] and vals[7] == b[1, 0, 0]
+    assert vals[35] == b[0, 0, 1]
+    import fusion_oracle as fo
+    assert np.array_equal(fo.read_array(tmp_path / "b.bin"), b)
+
+
+def test_workspace_layout(tiny):
+    mvs = tiny["mvs"]
+    for p in ("images", "sparse", "stereo/depth_maps", "stereo/normal_maps",
+              "stereo/consistency_graphs", "stereo/patch-match.cfg", "stereo/fusion.cfg",
+              "raw_depth.txt", "reparam_depth.txt", "last_rows.txt", "reference_plane.txt",
+              "depth_ranges.txt"):
+        assert os.path.exists(os.path.join(mvs, p)), p
+    assert os.path.islink(os.path.join(mvs, "images"))
+    cfg = open(os.path.join(mvs, "stereo", "patch-match.cfg")).read().split("\n")
+    assert cfg[0] == tiny["names"][0] and cfg[1] == "__auto__, 20"
+
+
+def test_patch_match_cfg_forms(tmp_path):
+    from vissatsatellitestereo_amd.workspace import parse_patch_match_cfg
+    p = tmp_path / "pm.cfg"
+    p.write_text("a.png\n__auto__, 8\nb.png\n__all__\nc.png\na.png, b.png\n")
+    assert parse_patch_match_cfg(p) == [("a.png", ("__auto__", 8)), ("b.png", ("__all__",)),
+                                        ("c.png", ["a.png", "b.png"])]
+
+
+def test_auto_source_selection(tiny):
+    from vissatsatellitestereo_amd.workspace import select_sources
+    s = select_sources(tiny["mvs"])
+    for ref in tiny["names"]:
+        assert ref not in s[ref]
+        assert 1 <= len(s[ref]) <= 3
+
+
+def test_synth_is_deterministic(tmp_path):
+    from vissatsatellitestereo_amd import synth
+    spec = synth.SceneSpec(width=48, height=40, n_views=2, n_boxes=2, n_points=100)
+    a = synth.build_site(spec, str(tmp_path / "a"))
+    b = synth.build_site(spec, str(tmp_path / "b"))
+    from PIL import Image
+    for n in ("0000.png", "0001.png"):
+        ia = np.array(Image.open(tmp_path / "a/colmap/sfm_perspective/images" / n))
+        ib = np.array(Image.open(tmp_path / "b/colmap/sfm_perspective/images" / n))
+        assert np.array_equal(ia, ib)
+    assert a["aoi"] == b["aoi"]
+
+
+def test_header_symbols_exported():
+    """The built library loads (no device call) and exports every function
+    include/vissat_mvs.h declares, and the binding lists each of them."""
+    from vissatsatellitestereo_amd import _lib
+    hdr = open(os.path.join(ROOT, "include", "vissat_mvs.h")).read()
+    declared = set(re.findall(r"^\s*(?:int|void|const char \*)\s*\*?\s*(vsm_\w+)\s*\(", hdr, re.M))
+    assert len(declared) >= 20
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for name in declared:
+        assert hasattr(lib, name), name
+    bound = {n for n, _, _ in _lib.SIGNATURES}
+    assert declared == bound, declared ^ bound
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """ctypes mirrors of the C structs have the sizes and field offsets the C
+    compiler gives for include/vissat_mvs.h."""
+    import subprocess
+    from vissatsatellitestereo_amd import _lib
+    checks = [("vsm_params", _lib.Params), ("vsm_view_req", _lib.ViewReq),
+              ("vsm_view_out", _lib.ViewOut), ("vsm_dsm_grid", _lib.DsmGrid), ("vsm_stats", _lib.Stats)]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "vissat_mvs.h"', "int main(void){"]
+    for cname, py in checks:
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    vals = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    k = 0
+    for cname, py in checks:
+        assert vals[k] == ctypes.sizeof(py), cname
+        k += 1
+        for f, _ in py._fields_:
+            assert vals[k] == getattr(py, f).offset, (cname, f)
+            k += 1
+
+
+def test_default_params_are_reference_flags():
+    from vissatsatellitestereo_amd import _lib
+    p = _lib.default_params()
+    assert p.window_radius == 3 and p.num_samples == 15 and p.num_iterations == 12
+    assert p.min_triangulation_angle == 5.0 and p.filter == 0 and p.geom_consistency == 0
+    assert abs(p.geom_consistency_regularizer - 0.3) < 1e-7 and p.geom_consistency_max_cost == 3.0
+    assert abs(p.filter_min_ncc - 0.1) < 1e-7 and abs(p.filter_min_triangulation_angle - 4.99) < 1e-6
+    assert p.filter_geom_consistency_max_cost == 1.0 and p.filter_min_num_consistent == 2
+
+
+def test_no_device_raises_loudly():
+    """Without a HIP device the product path fails instead of falling back."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    from vissatsatellitestereo_amd.mvs import MvsContext
+    from vissatsatellitestereo_amd._lib import VsmError
+    with pytest.raises(VsmError):
+        MvsContext(device=0)

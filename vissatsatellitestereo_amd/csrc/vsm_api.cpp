@@ -1,0 +1,1289 @@
+// vsm_api.cpp -- C ABI of libvissat_mvs.so (declared in include/vissat_mvs.h).
+//
+// Host side of the MVS path: context and HBM budget, image upload, FP64
+// geometry setup per reference view, batched red-black PatchMatch solves
+// (optionally replayed from a captured hipGraph), filter and outputs, the
+// serving loop (continuous batching with HBM-footprint admission) and 2.5D
+// DSM fusion.  Replaces the `colmap patch_match_stereo` subprocess of the
+// reference (colmap_mvs_commands.py:39-82, lib/run_cmd.py:44) and the Python
+// fusion (aggregate_2p5d.py:45-89).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/vissat_mvs.h"
+#include "vsm_device.h"
+#include "vsm_kernels.h"
+
+using namespace vsm;
+
+namespace {
+
+struct ImageRec {
+    int id = 0, W = 0, H = 0;
+    uint8_t *gray = nullptr;
+    uint32_t *quad = nullptr;
+    double P4[16];
+    double Pinv[16];
+    float *depth[2] = {nullptr, nullptr};   // photometric / geometric depth kept on device
+    float4 *hyp_photo = nullptr;            // photometric hypotheses (geometric init)
+};
+
+struct ViewState {
+    ViewDesc desc;                          // host copy
+    float4 *hyp = nullptr;
+    float *cost = nullptr;
+    float *prob = nullptr;
+    bool own_hyp = true;
+};
+
+struct ServeReq {
+    uint64_t ticket = 0;
+    int ref = 0;
+    std::vector<int> srcs;
+    float dmin = 0.f, dmax = 0.f;
+    // device state while active
+    ViewState vs;
+    int slot = -1;
+    int host_step = 0;
+    size_t footprint = 0;
+    // results (device) when done
+    float *out_depth = nullptr, *out_normal = nullptr, *out_cost = nullptr;
+    bool done = false;
+    int error = 0;
+    std::string errmsg;
+};
+
+}  // namespace
+
+struct vsm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0, in_use = 0, peak = 0;
+    vsm_params params;
+    std::map<int, ImageRec> images;
+    std::map<void *, size_t> allocs;
+    std::string err;
+    Counters *d_counters = nullptr;
+    bool timing = false;
+    double halfstep_ms = 0.0;
+    int64_t halfstep_launches = 0;
+    double fuse_ms = 0.0;
+    int64_t fuse_launches = 0, fuse_pixels = 0;
+    int64_t taps_per_eval = 0;
+    int64_t acc_ncc = 0, acc_geo = 0, acc_taps = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // serving
+    std::mutex serve_mu;
+    uint64_t next_ticket = 1;
+    std::deque<std::unique_ptr<ServeReq>> queue;
+    std::vector<ServeReq *> active;
+    std::map<uint64_t, std::unique_ptr<ServeReq>> reqs;
+    ViewDesc *serve_views = nullptr;
+    Tile *serve_tiles = nullptr;
+    int serve_cap_views = 0, serve_cap_tiles = 0;
+};
+
+namespace {
+
+constexpr int kServeMaxViews = 64;
+
+int fail(vsm_ctx *c, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                       \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail((ctx), VSM_EDEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                    \
+    } while (0)
+
+int dmalloc(vsm_ctx *c, void **p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) bytes = 16;
+    if (c->budget && c->in_use + bytes > c->budget)
+        return fail(c, VSM_ENOMEM, "HBM budget exceeded: %llu + %llu > %llu bytes",
+                    (unsigned long long)c->in_use, (unsigned long long)bytes,
+                    (unsigned long long)c->budget);
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(c, VSM_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    c->allocs[*p] = bytes;
+    c->in_use += bytes;
+    c->peak = std::max(c->peak, c->in_use);
+    return VSM_OK;
+}
+
+void dfree(vsm_ctx *c, void *p)
+{
+    if (!p) return;
+    auto it = c->allocs.find(p);
+    if (it != c->allocs.end()) {
+        c->in_use -= it->second;
+        c->allocs.erase(it);
+    }
+    (void)hipFree(p);
+}
+
+template <typename T>
+int dalloc(vsm_ctx *c, T **p, size_t count)
+{
+    return dmalloc(c, reinterpret_cast<void **>(p), count * sizeof(T));
+}
+
+// ---------------------------------------------------------------------------
+// FP64 linear algebra for the geometry setup
+// ---------------------------------------------------------------------------
+bool inv4(const double *a, double *out)
+{
+    double m[4][8];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            m[i][j] = a[4 * i + j];
+            m[i][4 + j] = i == j ? 1.0 : 0.0;
+        }
+    for (int c = 0; c < 4; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < 4; ++r)
+            if (std::fabs(m[r][c]) > std::fabs(m[piv][c])) piv = r;
+        if (m[piv][c] == 0.0) return false;
+        if (piv != c)
+            for (int j = 0; j < 8; ++j) std::swap(m[c][j], m[piv][j]);
+        const double d = m[c][c];
+        for (int j = 0; j < 8; ++j) m[c][j] /= d;
+        for (int r = 0; r < 4; ++r) {
+            if (r == c) continue;
+            const double f = m[r][c];
+            if (f == 0.0) continue;
+            for (int j = 0; j < 8; ++j) m[r][j] -= f * m[c][j];
+        }
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) out[4 * i + j] = m[i][4 + j];
+    return true;
+}
+
+bool inv3(const double *a, double *o)
+{
+    const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                       a[2] * (a[3] * a[7] - a[4] * a[6]);
+    if (det == 0.0 || !std::isfinite(det)) return false;
+    const double id = 1.0 / det;
+    o[0] = (a[4] * a[8] - a[5] * a[7]) * id;
+    o[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+    o[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+    o[3] = (a[5] * a[6] - a[3] * a[8]) * id;
+    o[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+    o[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+    o[6] = (a[3] * a[7] - a[4] * a[6]) * id;
+    o[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+    o[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+    return true;
+}
+
+// (3x4 rows of A) x (4x4 B), normalised to max |.| = 1.
+void mul34_44(const double *A, const double *B, double *out)
+{
+    double mx = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 4; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < 4; ++k) s += A[4 * i + k] * B[4 * k + j];
+            out[4 * i + j] = s;
+            mx = std::max(mx, std::fabs(s));
+        }
+    if (mx > 0.0)
+        for (int k = 0; k < 12; ++k) out[k] /= mx;
+}
+
+void camera_center(const double *P4, double C[3])
+{
+    double A[9] = {P4[0], P4[1], P4[2], P4[4], P4[5], P4[6], P4[8], P4[9], P4[10]}, Ai[9];
+    inv3(A, Ai);
+    for (int i = 0; i < 3; ++i)
+        C[i] = -(Ai[3 * i] * P4[3] + Ai[3 * i + 1] * P4[7] + Ai[3 * i + 2] * P4[11]);
+}
+
+double norm3(const double *v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+// Per-view geometry: M_s, Mb_s, triangulation angles/priors, g_s = J v_s,
+// J = d(x,y,d)/dX at the scene centre X0, and the normal frame maps.
+int build_geometry(vsm_ctx *c, const ImageRec &ref, const std::vector<const ImageRec *> &srcs,
+                   float dmin, float dmax, ViewDesc &V)
+{
+    const double dmid = 0.5 * ((double)dmin + (double)dmax);
+    const double p[4] = {(ref.W - 1) / 2.0, (ref.H - 1) / 2.0, 1.0, dmid};
+    double Xh[4];
+    for (int i = 0; i < 4; ++i)
+        Xh[i] = ref.Pinv[4 * i] * p[0] + ref.Pinv[4 * i + 1] * p[1] + ref.Pinv[4 * i + 2] * p[2] +
+                ref.Pinv[4 * i + 3] * p[3];
+    if (!(std::fabs(Xh[3]) > 0.0)) return fail(c, VSM_EINVAL, "degenerate P4 for image %d", ref.id);
+    const double X0[3] = {Xh[0] / Xh[3], Xh[1] / Xh[3], Xh[2] / Xh[3]};
+    double y[4];
+    for (int i = 0; i < 4; ++i)
+        y[i] = ref.P4[4 * i] * X0[0] + ref.P4[4 * i + 1] * X0[1] + ref.P4[4 * i + 2] * X0[2] +
+               ref.P4[4 * i + 3];
+    double J[9];
+    const int rows[3] = {0, 1, 3};
+    double mx = 0.0;
+    for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 3; ++k) {
+            const int i = rows[r];
+            J[3 * r + k] = (ref.P4[4 * i + k] * y[2] - y[i] * ref.P4[8 + k]) / (y[2] * y[2]);
+            mx = std::max(mx, std::fabs(J[3 * r + k]));
+        }
+    for (int k = 0; k < 9; ++k) J[k] /= mx;
+    double Cr[3];
+    camera_center(ref.P4, Cr);
+    const double amin = (double)c->params.min_triangulation_angle * M_PI / 180.0;
+    for (size_t s = 0; s < srcs.size(); ++s) {
+        const ImageRec &S = *srcs[s];
+        double M[12], Mb[12];
+        mul34_44(S.P4, ref.Pinv, M);
+        mul34_44(ref.P4, S.Pinv, Mb);
+        for (int k = 0; k < 12; ++k) { V.M[s][k] = M[k]; V.Mb[s][k] = Mb[k]; }
+        double Cs[3];
+        camera_center(S.P4, Cs);
+        const double a[3] = {Cr[0] - X0[0], Cr[1] - X0[1], Cr[2] - X0[2]};
+        const double b[3] = {Cs[0] - X0[0], Cs[1] - X0[1], Cs[2] - X0[2]};
+        const double na = norm3(a), nb = norm3(b);
+        double cs = (a[0] * b[0] + a[1] * b[1] + a[2] * b[2]) / (na * nb);
+        cs = std::min(1.0, std::max(-1.0, cs));
+        const double tri = std::acos(cs);
+        const double tm = std::min(tri, amin);
+        V.tri[s] = (float)tri;
+        V.ptri[s] = (float)(amin > 0.0 ? 1.0 - (tm - amin) * (tm - amin) / (amin * amin) : 1.0);
+        const double v[3] = {b[0] / nb, b[1] / nb, b[2] / nb};
+        for (int r = 0; r < 3; ++r)
+            V.g[s][r] = (float)(J[3 * r] * v[0] + J[3 * r + 1] * v[1] + J[3 * r + 2] * v[2]);
+        V.src_w[s] = S.W;
+        V.src_h[s] = S.H;
+        V.src_quad[s] = S.quad;
+        V.src_depth[s] = nullptr;
+    }
+    // Rotation of the reference camera from P[:3,:3] = K R.
+    const double *A = ref.P4;
+    double r3[3] = {A[8], A[9], A[10]};
+    double n3 = norm3(r3);
+    for (double &x : r3) x /= n3;
+    double r2[3] = {A[4], A[5], A[6]};
+    const double d23 = r2[0] * r3[0] + r2[1] * r3[1] + r2[2] * r3[2];
+    for (int k = 0; k < 3; ++k) r2[k] -= d23 * r3[k];
+    const double n2 = norm3(r2);
+    for (double &x : r2) x /= n2;
+    double r1[3] = {r2[1] * r3[2] - r2[2] * r3[1], r2[2] * r3[0] - r2[0] * r3[2],
+                    r2[0] * r3[1] - r2[1] * r3[0]};
+    if (r1[0] * A[0] + r1[1] * A[1] + r1[2] * A[2] < 0)
+        for (double &x : r1) x = -x;
+    const double R[9] = {r1[0], r1[1], r1[2], r2[0], r2[1], r2[2], r3[0], r3[1], r3[2]};
+    double Nout[9], Nin[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            Nout[3 * i + j] = R[3 * i] * J[3 * j] + R[3 * i + 1] * J[3 * j + 1] + R[3 * i + 2] * J[3 * j + 2];
+    if (!inv3(Nout, Nin)) return fail(c, VSM_EINVAL, "singular normal frame for image %d", ref.id);
+    for (int k = 0; k < 9; ++k) {
+        V.J[k] = (float)J[k];
+        V.N_out[k] = (float)Nout[k];
+        V.N_in[k] = (float)Nin[k];
+    }
+    V.depth_min = dmin;
+    V.depth_max = dmax;
+    return VSM_OK;
+}
+
+PMConst make_const(const vsm_params &p)
+{
+    PMConst c;
+    c.radius = p.window_radius;
+    c.step = p.window_step;
+    c.ntaps = 2 * p.window_radius / p.window_step + 1;
+    const double ss = p.sigma_spatial > 0 ? p.sigma_spatial : (double)p.window_radius;
+    const double log2e = 1.4426950408889634;
+    c.css = (float)(log2e / (2.0 * ss * ss));
+    const double sc = 255.0 * p.sigma_color;
+    c.ccc = (float)(log2e / (2.0 * sc * sc));
+    c.min_var = (float)(1e-5 * 255.0 * 255.0);
+    c.inv2n = (float)(1.0 / (2.0 * p.ncc_sigma * p.ncc_sigma));
+    c.inv2i = (float)(1.0 / (2.0 * p.incident_angle_sigma * p.incident_angle_sigma));
+    c.num_samples = p.num_samples;
+    c.geom_reg = p.geom_consistency_regularizer;
+    c.geom_max = p.geom_consistency_max_cost;
+    c.f_min_ncc = p.filter_min_ncc;
+    c.f_min_tri = (float)(p.filter_min_triangulation_angle * M_PI / 180.0);
+    c.f_geom_max = p.filter_geom_consistency_max_cost;
+    c.f_min_consistent = p.filter_min_num_consistent;
+    c.perturbation = p.perturbation;
+    c.normal_perturbation = p.normal_perturbation;
+    const double cm = std::cos((double)p.max_normal_angle * M_PI / 180.0);
+    c.cos_max_normal = (float)cm;
+    c.one_minus_cos_max = (float)(1.0 - cm);
+    c.decay_base = p.perturb_decay;
+    c.n_far = p.n_far;
+    c.seed_lo = (uint32_t)p.seed;
+    c.seed_hi = (uint32_t)(p.seed >> 32);
+    return c;
+}
+
+int check_params(vsm_ctx *c, const vsm_params &p)
+{
+    if (p.window_radius < 1 || p.window_radius > VSM_MAX_RADIUS)
+        return fail(c, VSM_EINVAL, "window_radius %d outside [1, %d]", p.window_radius, VSM_MAX_RADIUS);
+    if (p.window_step < 1 || p.window_step > p.window_radius)
+        return fail(c, VSM_EINVAL, "window_step %d outside [1, window_radius]", p.window_step);
+    if (p.num_samples < 1 || p.num_samples > kMaxSamples)
+        return fail(c, VSM_EINVAL, "num_samples %d outside [1, %d]", p.num_samples, kMaxSamples);
+    if (p.num_iterations < 0 || p.num_iterations > 1000)
+        return fail(c, VSM_EINVAL, "num_iterations %d outside [0, 1000]", p.num_iterations);
+    if (!(p.sigma_color > 0) || !(p.ncc_sigma > 0) || !(p.incident_angle_sigma > 0))
+        return fail(c, VSM_EINVAL, "sigma_color, ncc_sigma and incident_angle_sigma must be > 0");
+    if (p.n_far < 1 || p.n_far > 64) return fail(c, VSM_EINVAL, "n_far %d outside [1, 64]", p.n_far);
+    if (!(p.max_normal_angle > 0 && p.max_normal_angle < 90))
+        return fail(c, VSM_EINVAL, "max_normal_angle must be in (0, 90) degrees");
+    return VSM_OK;
+}
+
+void tiles_for(const ViewDesc &V, int view_index, std::vector<Tile> &out)
+{
+    for (int y0 = 0; y0 < V.H; y0 += kTileH)
+        for (int x0 = 0; x0 < V.W; x0 += kTileW) out.push_back(Tile{view_index, x0, y0, 0});
+}
+
+size_t state_bytes(int W, int H, int S)
+{
+    const size_t hw = (size_t)W * H;
+    return hw * sizeof(float4) + hw * sizeof(float) + hw * S * sizeof(float);
+}
+
+// Prepare one view: validation, geometry, state allocation.
+int prepare_view(vsm_ctx *c, const vsm_view_req &r, int pass, uint32_t view_id, ViewState &vs)
+{
+    auto it = c->images.find(r.ref_id);
+    if (it == c->images.end()) return fail(c, VSM_ENOENT, "unknown reference image %d", r.ref_id);
+    const ImageRec &ref = it->second;
+    if (!ref.quad) return fail(c, VSM_EINVAL, "image %d was registered without pixels", r.ref_id);
+    if (r.n_src < 1 || r.n_src > VSM_MAX_SRC)
+        return fail(c, VSM_EINVAL, "n_src %d outside [1, %d]", r.n_src, VSM_MAX_SRC);
+    if (!r.src_ids) return fail(c, VSM_EINVAL, "src_ids is NULL");
+    std::vector<const ImageRec *> srcs;
+    for (int s = 0; s < r.n_src; ++s) {
+        auto si = c->images.find(r.src_ids[s]);
+        if (si == c->images.end()) return fail(c, VSM_ENOENT, "unknown source image %d", r.src_ids[s]);
+        if (r.src_ids[s] == r.ref_id) return fail(c, VSM_EINVAL, "source %d is the reference", r.ref_id);
+        if (!si->second.quad) return fail(c, VSM_EINVAL, "image %d was registered without pixels", r.src_ids[s]);
+        srcs.push_back(&si->second);
+    }
+    float dmin = c->params.depth_min > 0 ? c->params.depth_min : r.depth_min;
+    float dmax = c->params.depth_max > 0 ? c->params.depth_max : r.depth_max;
+    if (!(dmax > dmin) || !std::isfinite(dmin) || !std::isfinite(dmax))
+        return fail(c, VSM_EINVAL, "invalid depth range [%g, %g] for image %d", dmin, dmax, r.ref_id);
+    memset(&vs.desc, 0, sizeof(ViewDesc));
+    ViewDesc &V = vs.desc;
+    int rc = build_geometry(c, ref, srcs, dmin, dmax, V);
+    if (rc) return rc;
+    V.W = ref.W;
+    V.H = ref.H;
+    V.n_src = r.n_src;
+    V.pass = pass;
+    V.step = 0;
+    V.num_steps = 2 * c->params.num_iterations;
+    V.view_id = view_id;
+    V.active = 1;
+    V.ref = ref.gray;
+    if (pass == 1) {
+        for (int s = 0; s < r.n_src; ++s) {
+            const ImageRec &S = *srcs[s];
+            if (!S.depth[0])
+                return fail(c, VSM_EINVAL,
+                            "geometric pass needs the photometric depth map of source image %d",
+                            S.id);
+            V.src_depth[s] = S.depth[0];
+        }
+    }
+    const size_t hw = (size_t)ref.W * ref.H;
+    rc = dalloc(c, &vs.hyp, hw);
+    if (!rc) rc = dalloc(c, &vs.cost, hw);
+    if (!rc) rc = dalloc(c, &vs.prob, hw * r.n_src);
+    if (rc) {
+        dfree(c, vs.hyp); dfree(c, vs.cost); dfree(c, vs.prob);
+        vs.hyp = nullptr; vs.cost = nullptr; vs.prob = nullptr;
+        return rc;
+    }
+    V.hyp = vs.hyp;
+    V.cost = vs.cost;
+    V.prob = vs.prob;
+    return VSM_OK;
+}
+
+void release_view(vsm_ctx *c, ViewState &vs)
+{
+    if (vs.own_hyp) dfree(c, vs.hyp);
+    dfree(c, vs.cost);
+    dfree(c, vs.prob);
+    vs.hyp = nullptr; vs.cost = nullptr; vs.prob = nullptr;
+}
+
+int record_halfstep(vsm_ctx *c, int maxs, const ViewDesc *dviews, const Tile *dtiles, int ntiles,
+                    const PMConst &pc, bool timed)
+{
+    if (timed) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, launch_halfstep(maxs, dviews, dtiles, ntiles, pc, c->d_counters, c->stream));
+    if (timed) {
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+        HIPCHK(c, hipEventSynchronize(c->ev1));
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->halfstep_ms += ms;
+    }
+    c->halfstep_launches += 1;
+    return VSM_OK;
+}
+
+int read_counters(vsm_ctx *c)
+{
+    Counters h;
+    HIPCHK(c, hipMemcpyAsync(&h, c->d_counters, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, launch_fill_u64(reinterpret_cast<unsigned long long *>(c->d_counters), 4, 0ull, c->stream));
+    const PMConst pc = make_const(c->params);
+    c->acc_ncc += (int64_t)h.ncc_evals;
+    c->acc_geo += (int64_t)h.geo_evals;
+    c->acc_taps += (int64_t)h.ncc_evals * pc.ntaps * pc.ntaps;
+    return VSM_OK;
+}
+
+// Batched solve core: views already prepared; runs init/iterations/filter.
+int run_views(vsm_ctx *c, std::vector<ViewState> &views, const std::vector<const vsm_view_req *> &reqs,
+              int pass, uint32_t *d_masks, const std::vector<size_t> &mask_off, int use_graph)
+{
+    const int n = (int)views.size();
+    const PMConst pc = make_const(c->params);
+    int maxs = 1;
+    std::vector<Tile> tiles;
+    for (int v = 0; v < n; ++v) {
+        maxs = std::max(maxs, views[v].desc.n_src);
+        tiles_for(views[v].desc, v, tiles);
+    }
+    ViewDesc *dviews = nullptr;
+    Tile *dtiles = nullptr;
+    int rc = dalloc(c, &dviews, n);
+    if (!rc) rc = dalloc(c, &dtiles, tiles.size());
+    if (rc) { dfree(c, dviews); return rc; }
+    std::vector<ViewDesc> hv(n);
+    for (int v = 0; v < n; ++v) hv[v] = views[v].desc;
+    auto cleanup = [&]() { dfree(c, dviews); dfree(c, dtiles); };
+#define RUNCHK(expr)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            cleanup();                                                                          \
+            return fail(c, VSM_EDEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));        \
+        }                                                                                       \
+    } while (0)
+    RUNCHK(hipMemcpyAsync(dviews, hv.data(), n * sizeof(ViewDesc), hipMemcpyHostToDevice, c->stream));
+    RUNCHK(hipMemcpyAsync(dtiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice,
+                          c->stream));
+    const int ntiles = (int)tiles.size();
+    // initial hypotheses
+    if (pass == 0) {
+        RUNCHK(launch_init(dviews, dtiles, ntiles, pc, 1, c->stream));
+    } else {
+        for (int v = 0; v < n; ++v) {
+            const vsm_view_req &r = *reqs[v];
+            const ImageRec &ref = c->images[r.ref_id];
+            const size_t hw = (size_t)ref.W * ref.H;
+            if (ref.hyp_photo) {
+                RUNCHK(hipMemcpyAsync(views[v].hyp, ref.hyp_photo, hw * sizeof(float4),
+                                      hipMemcpyDeviceToDevice, c->stream));
+            } else if (r.init_depth && r.init_normal) {
+                float *tmp = nullptr;
+                if ((rc = dalloc(c, &tmp, hw * 4))) { cleanup(); return rc; }
+                RUNCHK(hipMemcpyAsync(tmp, r.init_depth, hw * 4, hipMemcpyHostToDevice, c->stream));
+                RUNCHK(hipMemcpyAsync(tmp + hw, r.init_normal, hw * 12, hipMemcpyHostToDevice, c->stream));
+                RUNCHK(launch_load_hyp(dviews, v, (int)hw, tmp, tmp + hw, c->stream));
+                RUNCHK(hipStreamSynchronize(c->stream));
+                dfree(c, tmp);
+            } else {
+                cleanup();
+                return fail(c, VSM_EINVAL,
+                            "geometric pass of image %d needs its photometric result (solve pass 0 "
+                            "first or pass init_depth/init_normal)", r.ref_id);
+            }
+        }
+        RUNCHK(launch_init(dviews, dtiles, ntiles, pc, 0, c->stream));
+    }
+    const int nsteps = 2 * c->params.num_iterations;
+    if (use_graph && nsteps > 0) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        RUNCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t ce = hipSuccess;
+        for (int s = 0; s < nsteps && ce == hipSuccess; ++s) {
+            ce = launch_halfstep(maxs, dviews, dtiles, ntiles, pc, c->d_counters, c->stream);
+            if (ce == hipSuccess) ce = launch_advance(dviews, n, c->stream);
+        }
+        hipError_t ee = hipStreamEndCapture(c->stream, &graph);
+        if (ce != hipSuccess || ee != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            cleanup();
+            return fail(c, VSM_EDEVICE, "graph capture failed: %s",
+                        hipGetErrorString(ce != hipSuccess ? ce : ee));
+        }
+        RUNCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        if (c->timing) RUNCHK(hipEventRecord(c->ev0, c->stream));
+        RUNCHK(hipGraphLaunch(exec, c->stream));
+        if (c->timing) {
+            RUNCHK(hipEventRecord(c->ev1, c->stream));
+            RUNCHK(hipEventSynchronize(c->ev1));
+            float ms = 0.f;
+            RUNCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            c->halfstep_ms += ms;
+        }
+        c->halfstep_launches += nsteps;
+        RUNCHK(hipStreamSynchronize(c->stream));
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+    } else {
+        for (int s = 0; s < nsteps; ++s) {
+            if ((rc = record_halfstep(c, maxs, dviews, dtiles, ntiles, pc, c->timing))) {
+                cleanup();
+                return rc;
+            }
+            RUNCHK(launch_advance(dviews, n, c->stream));
+        }
+    }
+    if (c->params.filter) {
+        for (int v = 0; v < n; ++v) {
+            // per-view launch so the mask buffer offsets stay simple
+            std::vector<Tile> vt;
+            tiles_for(views[v].desc, 0, vt);
+            Tile *dt = nullptr;
+            if ((rc = dalloc(c, &dt, vt.size()))) { cleanup(); return rc; }
+            RUNCHK(hipMemcpyAsync(dt, vt.data(), vt.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream));
+            RUNCHK(launch_filter(maxs, dviews + v, dt, (int)vt.size(), pc,
+                                 d_masks ? d_masks + mask_off[v] : nullptr, c->stream));
+            RUNCHK(hipStreamSynchronize(c->stream));
+            dfree(c, dt);
+        }
+    }
+    RUNCHK(hipStreamSynchronize(c->stream));
+    cleanup();
+#undef RUNCHK
+    return read_counters(c);
+}
+
+int finalize_outputs(vsm_ctx *c, ViewState &vs, int pass, int ref_id, vsm_view_out *out,
+                     const uint32_t *d_mask)
+{
+    ImageRec &ref = c->images[ref_id];
+    const size_t hw = (size_t)ref.W * ref.H;
+    ViewDesc *dv = nullptr;
+    float *tmp = nullptr;
+    int rc = dalloc(c, &dv, 1);
+    if (!rc) rc = dalloc(c, &tmp, hw * 5);
+    if (rc) { dfree(c, dv); return rc; }
+    if (!ref.depth[pass] && (rc = dalloc(c, &ref.depth[pass], hw))) { dfree(c, dv); dfree(c, tmp); return rc; }
+    HIPCHK(c, hipMemcpyAsync(dv, &vs.desc, sizeof(ViewDesc), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_finalize(dv, 0, (int)hw, tmp, tmp + hw, tmp + 4 * hw, ref.depth[pass], c->stream));
+    if (out && out->depth)
+        HIPCHK(c, hipMemcpyAsync(out->depth, tmp, hw * 4, hipMemcpyDeviceToHost, c->stream));
+    if (out && out->normal)
+        HIPCHK(c, hipMemcpyAsync(out->normal, tmp + hw, hw * 12, hipMemcpyDeviceToHost, c->stream));
+    if (out && out->cost)
+        HIPCHK(c, hipMemcpyAsync(out->cost, tmp + 4 * hw, hw * 4, hipMemcpyDeviceToHost, c->stream));
+    if (out && out->consistency) {
+        if (d_mask) HIPCHK(c, hipMemcpyAsync(out->consistency, d_mask, hw * 4, hipMemcpyDeviceToHost, c->stream));
+        else memset(out->consistency, 0, hw * 4);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(c, dv);
+    dfree(c, tmp);
+    if (pass == 0) {
+        // keep the photometric hypotheses for a later geometric pass
+        if (ref.hyp_photo) dfree(c, ref.hyp_photo);
+        ref.hyp_photo = vs.hyp;
+        vs.own_hyp = false;
+    }
+    return VSM_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Public C ABI
+// ===========================================================================
+extern "C" {
+
+void vsm_default_params(vsm_params *p)
+{
+    if (!p) return;
+    memset(p, 0, sizeof *p);
+    p->window_radius = 3;             // stereo_pipeline.py:444
+    p->window_step = 1;
+    p->sigma_spatial = -1.f;
+    p->sigma_color = 0.2f;
+    p->ncc_sigma = 0.6f;
+    p->num_samples = 15;              // colmap_mvs_commands.py:46
+    p->num_iterations = 12;           // colmap_mvs_commands.py:47
+    p->min_triangulation_angle = 5.f; // colmap_mvs_commands.py:42
+    p->incident_angle_sigma = 0.9f;
+    p->geom_consistency = 0;          // colmap_mvs_commands.py:44
+    p->geom_consistency_regularizer = 0.3f;    // colmap_mvs_commands.py:66
+    p->geom_consistency_max_cost = 3.f;        // colmap_mvs_commands.py:67
+    p->filter = 0;                             // colmap_mvs_commands.py:43
+    p->filter_min_ncc = 0.1f;                  // colmap_mvs_commands.py:70
+    p->filter_min_triangulation_angle = 4.99f; // colmap_mvs_commands.py:69
+    p->filter_geom_consistency_max_cost = 1.f; // colmap_mvs_commands.py:71
+    p->filter_min_num_consistent = 2;          // colmap_mvs_commands.py:72
+    p->depth_min = -1.f;
+    p->depth_max = -1.f;
+    p->perturbation = 0.05f;
+    p->normal_perturbation = 0.3f;
+    p->max_normal_angle = 80.f;
+    p->perturb_decay = 0.5f;
+    p->n_far = 6;
+    p->seed = 0x5A7E11173Dull;
+}
+
+int vsm_create(int device, uint64_t hbm_budget_bytes, vsm_ctx **out)
+{
+    if (!out) return VSM_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return VSM_EDEVICE;
+    if (device < 0 || device >= ndev) return VSM_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return VSM_EDEVICE;
+    auto *c = new vsm_ctx();
+    c->device = device;
+    c->budget = hbm_budget_bytes;
+    vsm_default_params(&c->params);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return VSM_EDEVICE;
+    }
+    if (dalloc(c, &c->d_counters, 1) != VSM_OK) {
+        delete c;
+        return VSM_ENOMEM;
+    }
+    (void)launch_fill_u64(reinterpret_cast<unsigned long long *>(c->d_counters), 4, 0ull, c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    *out = c;
+    return VSM_OK;
+}
+
+int vsm_destroy(vsm_ctx *c)
+{
+    if (!c) return VSM_EINVAL;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &kv : c->allocs) (void)hipFree(kv.first);
+    c->allocs.clear();
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return VSM_OK;
+}
+
+const char *vsm_last_error(vsm_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int vsm_set_params(vsm_ctx *c, const vsm_params *p)
+{
+    if (!c || !p) return VSM_EINVAL;
+    int rc = check_params(c, *p);
+    if (rc) return rc;
+    c->params = *p;
+    return VSM_OK;
+}
+
+int vsm_get_params(vsm_ctx *c, vsm_params *p)
+{
+    if (!c || !p) return VSM_EINVAL;
+    *p = c->params;
+    return VSM_OK;
+}
+
+int vsm_add_image(vsm_ctx *c, int32_t id, const uint8_t *gray, int32_t w, int32_t h, int32_t stride,
+                  const double P4[16])
+{
+    if (!c) return VSM_EINVAL;
+    if (!P4) return fail(c, VSM_EINVAL, "NULL P4");
+    if (w < 2 || h < 2 || w > 32768 || h > 32768 || (int64_t)w * h > (1ll << 30))
+        return fail(c, VSM_EINVAL, "image size %dx%d unsupported", w, h);
+    if (stride < w) return fail(c, VSM_EINVAL, "row_stride %d < width %d", stride, w);
+    for (int k = 0; k < 16; ++k)
+        if (!std::isfinite(P4[k])) return fail(c, VSM_EINVAL, "P4 has non-finite entries");
+    HIPCHK(c, hipSetDevice(c->device));
+    vsm_remove_image(c, id);
+    ImageRec r;
+    r.id = id;
+    r.W = w;
+    r.H = h;
+    memcpy(r.P4, P4, sizeof r.P4);
+    if (!inv4(P4, r.Pinv)) return fail(c, VSM_EINVAL, "P4 of image %d is singular", id);
+    if (gray) {
+        int rc = dalloc(c, &r.gray, (size_t)w * h);
+        if (!rc) rc = dalloc(c, &r.quad, (size_t)w * h);
+        if (rc) { dfree(c, r.gray); return rc; }
+        HIPCHK(c, hipMemcpy2DAsync(r.gray, w, gray, stride, w, h, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, launch_pack_quad(r.gray, w, h, r.quad, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    c->images[id] = r;
+    return VSM_OK;
+}
+
+int vsm_remove_image(vsm_ctx *c, int32_t id)
+{
+    if (!c) return VSM_EINVAL;
+    auto it = c->images.find(id);
+    if (it == c->images.end()) return VSM_ENOENT;
+    (void)hipStreamSynchronize(c->stream);
+    ImageRec &r = it->second;
+    dfree(c, r.gray);
+    dfree(c, r.quad);
+    dfree(c, r.depth[0]);
+    dfree(c, r.depth[1]);
+    dfree(c, r.hyp_photo);
+    c->images.erase(it);
+    return VSM_OK;
+}
+
+int vsm_set_depth_map(vsm_ctx *c, int32_t id, int32_t kind, const float *depth, int32_t w, int32_t h)
+{
+    if (!c || !depth) return VSM_EINVAL;
+    if (kind != 0 && kind != 1) return fail(c, VSM_EINVAL, "depth map kind must be 0 or 1");
+    auto it = c->images.find(id);
+    if (it == c->images.end()) return fail(c, VSM_ENOENT, "unknown image %d", id);
+    ImageRec &r = it->second;
+    if (w != r.W || h != r.H) return fail(c, VSM_EINVAL, "depth map %dx%d != image %dx%d", w, h, r.W, r.H);
+    if (!r.depth[kind]) {
+        int rc = dalloc(c, &r.depth[kind], (size_t)w * h);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipMemcpyAsync(r.depth[kind], depth, (size_t)w * h * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VSM_OK;
+}
+
+int vsm_solve(vsm_ctx *c, const vsm_view_req *reqs, int32_t n, int32_t pass, vsm_view_out *outs,
+              int32_t use_graph)
+{
+    if (!c) return VSM_EINVAL;
+    if (!reqs || n <= 0) return fail(c, VSM_EINVAL, "no requests");
+    if (pass != 0 && pass != 1) return fail(c, VSM_EINVAL, "pass must be 0 or 1");
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<ViewState> views(n);
+    std::vector<const vsm_view_req *> rv(n);
+    int rc = VSM_OK;
+    int prepared = 0;
+    for (int v = 0; v < n && !rc; ++v) {
+        rv[v] = &reqs[v];
+        rc = prepare_view(c, reqs[v], pass, (uint32_t)reqs[v].ref_id, views[v]);
+        if (!rc) ++prepared;
+    }
+    uint32_t *d_masks = nullptr;
+    std::vector<size_t> mask_off(n, 0);
+    if (!rc && c->params.filter) {
+        size_t tot = 0;
+        for (int v = 0; v < n; ++v) { mask_off[v] = tot; tot += (size_t)views[v].desc.W * views[v].desc.H; }
+        rc = dalloc(c, &d_masks, tot);
+    }
+    if (!rc) rc = run_views(c, views, rv, pass, d_masks, mask_off, use_graph);
+    for (int v = 0; v < n && !rc; ++v)
+        rc = finalize_outputs(c, views[v], pass, reqs[v].ref_id, outs ? &outs[v] : nullptr,
+                              d_masks ? d_masks + mask_off[v] : nullptr);
+    for (int v = 0; v < prepared; ++v) release_view(c, views[v]);
+    dfree(c, d_masks);
+    return rc;
+}
+
+int vsm_eval_cost(vsm_ctx *c, const vsm_view_req *req, int32_t pass, const float *depth,
+                  const float *normal, float *out)
+{
+    if (!c || !req || !depth || !normal || !out) return VSM_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    ViewState vs;
+    int rc = prepare_view(c, *req, pass, (uint32_t)req->ref_id, vs);
+    if (rc) return rc;
+    const ViewDesc &V = vs.desc;
+    const size_t hw = (size_t)V.W * V.H;
+    std::vector<float4> h(hw);
+    for (size_t i = 0; i < hw; ++i)
+        h[i] = make_float4(depth[i], normal[3 * i], normal[3 * i + 1], normal[3 * i + 2]);
+    std::vector<Tile> tiles;
+    tiles_for(V, 0, tiles);
+    ViewDesc *dv = nullptr;
+    Tile *dt = nullptr;
+    float *dout = nullptr;
+    rc = dalloc(c, &dv, 1);
+    if (!rc) rc = dalloc(c, &dt, tiles.size());
+    if (!rc) rc = dalloc(c, &dout, hw * V.n_src);
+    if (!rc) {
+        HIPCHK(c, hipMemcpyAsync(vs.hyp, h.data(), hw * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(dv, &V, sizeof V, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(dt, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, launch_eval_cost(V.n_src, dv, dt, (int)tiles.size(), make_const(c->params), dout, c->stream));
+        HIPCHK(c, hipMemcpyAsync(out, dout, hw * V.n_src * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    dfree(c, dv); dfree(c, dt); dfree(c, dout);
+    release_view(c, vs);
+    return rc;
+}
+
+static int debug_state(vsm_ctx *c, const vsm_view_req *req, int pass, int mode, int iter, int color,
+                       float *depth, float *normal, float *cost, float *prob, uint32_t *mask)
+{
+    if (!c || !req || !depth || !normal) return VSM_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    ViewState vs;
+    int rc = prepare_view(c, *req, pass, (uint32_t)req->ref_id, vs);
+    if (rc) return rc;
+    ViewDesc &V = vs.desc;
+    const size_t hw = (size_t)V.W * V.H;
+    V.step = 2 * iter + color;
+    V.num_steps = V.step + 1;
+    std::vector<Tile> tiles;
+    tiles_for(V, 0, tiles);
+    ViewDesc *dv = nullptr;
+    Tile *dt = nullptr;
+    uint32_t *dm = nullptr;
+    rc = dalloc(c, &dv, 1);
+    if (!rc) rc = dalloc(c, &dt, tiles.size());
+    if (!rc && mode == 2) rc = dalloc(c, &dm, hw);
+    const PMConst pc = make_const(c->params);
+    std::vector<float4> h(hw);
+    if (!rc) {
+        if (mode != 0) {
+            for (size_t i = 0; i < hw; ++i)
+                h[i] = make_float4(depth[i], normal[3 * i], normal[3 * i + 1], normal[3 * i + 2]);
+            HIPCHK(c, hipMemcpyAsync(vs.hyp, h.data(), hw * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+            if (cost) HIPCHK(c, hipMemcpyAsync(vs.cost, cost, hw * 4, hipMemcpyHostToDevice, c->stream));
+            if (prob) HIPCHK(c, hipMemcpyAsync(vs.prob, prob, hw * V.n_src * 4, hipMemcpyHostToDevice, c->stream));
+        }
+        HIPCHK(c, hipMemcpyAsync(dv, &V, sizeof V, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(dt, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream));
+        const int nt = (int)tiles.size();
+        if (mode == 0) HIPCHK(c, launch_init(dv, dt, nt, pc, 1, c->stream));
+        else if (mode == 1) HIPCHK(c, launch_halfstep(V.n_src, dv, dt, nt, pc, c->d_counters, c->stream));
+        else HIPCHK(c, launch_filter(V.n_src, dv, dt, nt, pc, dm, c->stream));
+        HIPCHK(c, hipMemcpyAsync(h.data(), vs.hyp, hw * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+        if (cost && mode != 2) HIPCHK(c, hipMemcpyAsync(cost, vs.cost, hw * 4, hipMemcpyDeviceToHost, c->stream));
+        if (prob && mode != 2) HIPCHK(c, hipMemcpyAsync(prob, vs.prob, hw * V.n_src * 4, hipMemcpyDeviceToHost, c->stream));
+        if (mask && mode == 2) HIPCHK(c, hipMemcpyAsync(mask, dm, hw * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (size_t i = 0; i < hw; ++i) {
+            depth[i] = h[i].x;
+            normal[3 * i] = h[i].y; normal[3 * i + 1] = h[i].z; normal[3 * i + 2] = h[i].w;
+        }
+        if (mode == 1) rc = read_counters(c);
+    }
+    dfree(c, dv); dfree(c, dt); dfree(c, dm);
+    release_view(c, vs);
+    return rc;
+}
+
+int vsm_debug_init(vsm_ctx *c, const vsm_view_req *req, int32_t pass, float *depth, float *normal,
+                   float *cost, float *prob)
+{
+    return debug_state(c, req, pass, 0, 0, 0, depth, normal, cost, prob, nullptr);
+}
+
+int vsm_debug_halfstep(vsm_ctx *c, const vsm_view_req *req, int32_t pass, int32_t iter, int32_t color,
+                       float *depth, float *normal, float *cost, float *prob)
+{
+    if (!cost || !prob) return VSM_EINVAL;
+    if (iter < 0 || color < 0 || color > 1) return VSM_EINVAL;
+    return debug_state(c, req, pass, 1, iter, color, depth, normal, cost, prob, nullptr);
+}
+
+int vsm_debug_filter(vsm_ctx *c, const vsm_view_req *req, int32_t pass, float *depth, float *normal,
+                     uint32_t *mask)
+{
+    return debug_state(c, req, pass, 2, 0, 0, depth, normal, nullptr, nullptr, mask);
+}
+
+// ---------------------------------------------------------------------------
+// Fusion
+// ---------------------------------------------------------------------------
+static DsmGridDev grid_dev(const vsm_dsm_grid &g)
+{
+    DsmGridDev d;
+    d.ul_e = g.ul_e; d.ul_n = g.ul_n; d.e_res = g.e_res; d.n_res = g.n_res;
+    d.e_size = g.e_size; d.n_size = g.n_size;
+    return d;
+}
+
+static int check_grid(vsm_ctx *c, const vsm_dsm_grid *g)
+{
+    if (!g) return fail(c, VSM_EINVAL, "NULL grid");
+    if (g->e_size <= 0 || g->n_size <= 0 || (int64_t)g->e_size * g->n_size > (1ll << 31))
+        return fail(c, VSM_EINVAL, "grid size %dx%d unsupported", g->e_size, g->n_size);
+    if (!(g->e_res > 0) || !(g->n_res > 0)) return fail(c, VSM_EINVAL, "grid resolution must be > 0");
+    return VSM_OK;
+}
+
+int vsm_fuse_dsm(vsm_ctx *c, const vsm_dsm_grid *grid, const int32_t *ids, int32_t n,
+                 int32_t depth_kind, int32_t min_measurements, const double *inv_proj_mats,
+                 float *dsm_out, float *per_view_out)
+{
+    if (!c) return VSM_EINVAL;
+    int rc = check_grid(c, grid);
+    if (rc) return rc;
+    if (!ids || n <= 0 || n > 64) return fail(c, VSM_EINVAL, "1..64 images required, got %d", n);
+    if (depth_kind != 0 && depth_kind != 1) return fail(c, VSM_EINVAL, "depth_kind must be 0 or 1");
+    if (!dsm_out) return fail(c, VSM_EINVAL, "NULL dsm_out");
+    for (int v = 0; v < n; ++v) {
+        auto it = c->images.find(ids[v]);
+        if (it == c->images.end()) return fail(c, VSM_ENOENT, "unknown image %d", ids[v]);
+        if (!it->second.depth[depth_kind])
+            return fail(c, VSM_EINVAL, "image %d has no %s depth map", ids[v],
+                        depth_kind ? "geometric" : "photometric");
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    const DsmGridDev g = grid_dev(*grid);
+    const long long ncell = (long long)g.e_size * g.n_size;
+    unsigned long long *cells = nullptr;
+    float *stack = nullptr, *tmp = nullptr, *fused = nullptr;
+    double *mats = nullptr;
+    rc = dalloc(c, &cells, ncell);
+    if (!rc) rc = dalloc(c, &stack, ncell * n);
+    if (!rc) rc = dalloc(c, &tmp, ncell);
+    if (!rc) rc = dalloc(c, &fused, ncell);
+    if (!rc) rc = dalloc(c, &mats, 16 * (size_t)n);
+    if (rc) { dfree(c, cells); dfree(c, stack); dfree(c, tmp); dfree(c, fused); return rc; }
+    std::vector<double> hm(16 * (size_t)n);
+    for (int v = 0; v < n; ++v)
+        memcpy(&hm[16 * v], inv_proj_mats ? inv_proj_mats + 16 * v : c->images[ids[v]].Pinv,
+               16 * sizeof(double));
+    HIPCHK(c, hipMemcpyAsync(mats, hm.data(), hm.size() * 8, hipMemcpyHostToDevice, c->stream));
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    int64_t pixels = 0;
+    for (int v = 0; v < n; ++v) {
+        const ImageRec &r = c->images[ids[v]];
+        HIPCHK(c, launch_fill_u64n(cells, ncell, 0ull, c->stream));
+        HIPCHK(c, launch_rasterize_view(r.depth[depth_kind], r.W, r.H, mats + 16 * v, g, cells, c->stream));
+        HIPCHK(c, launch_decode_fill_f32(cells, g.e_size, g.n_size, tmp, c->stream));
+        HIPCHK(c, launch_median3(tmp, g.e_size, g.n_size, stack + (size_t)v * ncell, c->stream));
+        pixels += (int64_t)r.W * r.H;
+    }
+    HIPCHK(c, launch_fuse_stack(stack, n, ncell, min_measurements, tmp, c->stream));
+    HIPCHK(c, launch_median3(tmp, g.e_size, g.n_size, fused, c->stream));
+    if (c->timing) {
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+        HIPCHK(c, hipEventSynchronize(c->ev1));
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->fuse_ms += ms;
+    }
+    c->fuse_launches += 1;
+    c->fuse_pixels += pixels;
+    HIPCHK(c, hipMemcpyAsync(dsm_out, fused, ncell * 4, hipMemcpyDeviceToHost, c->stream));
+    if (per_view_out)
+        HIPCHK(c, hipMemcpyAsync(per_view_out, stack, ncell * 4 * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(c, cells); dfree(c, stack); dfree(c, tmp); dfree(c, fused); dfree(c, mats);
+    return VSM_OK;
+}
+
+int vsm_proj_to_grid(vsm_ctx *c, const double *points, int64_t n, const vsm_dsm_grid *grid,
+                     double *dsm_out)
+{
+    if (!c) return VSM_EINVAL;
+    int rc = check_grid(c, grid);
+    if (rc) return rc;
+    if ((n > 0 && !points) || !dsm_out || n < 0) return fail(c, VSM_EINVAL, "bad points/output");
+    HIPCHK(c, hipSetDevice(c->device));
+    const DsmGridDev g = grid_dev(*grid);
+    const long long ncell = (long long)g.e_size * g.n_size;
+    unsigned long long *cells = nullptr;
+    double *pts = nullptr, *out = nullptr;
+    rc = dalloc(c, &cells, ncell);
+    if (!rc) rc = dalloc(c, &out, ncell);
+    if (!rc && n > 0) rc = dalloc(c, &pts, 3 * (size_t)n);
+    if (rc) { dfree(c, cells); dfree(c, out); return rc; }
+    if (n > 0) HIPCHK(c, hipMemcpyAsync(pts, points, 24 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_fill_u64n(cells, ncell, 0ull, c->stream));
+    HIPCHK(c, launch_rasterize_points(pts, n, g, cells, c->stream));
+    HIPCHK(c, launch_decode_fill_f64(cells, g.e_size, g.n_size, out, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dsm_out, out, ncell * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(c, cells); dfree(c, out); dfree(c, pts);
+    return VSM_OK;
+}
+
+int vsm_median_blur3(vsm_ctx *c, const float *in, int32_t e_size, int32_t n_size, float *out)
+{
+    if (!c) return VSM_EINVAL;
+    if (!in || !out || e_size <= 0 || n_size <= 0 || (int64_t)e_size * n_size > (1ll << 31))
+        return fail(c, VSM_EINVAL, "bad raster %dx%d", e_size, n_size);
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t nc = (size_t)e_size * n_size;
+    float *buf = nullptr;
+    int rc = dalloc(c, &buf, 2 * nc);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(buf, in, nc * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_median3(buf, e_size, n_size, buf + nc, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out, buf + nc, nc * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(c, buf);
+    return VSM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Serving: continuous batching of photometric solves.
+// ---------------------------------------------------------------------------
+static size_t serve_footprint(vsm_ctx *c, const ServeReq &r)
+{
+    const ImageRec &ref = c->images[r.ref];
+    const size_t hw = (size_t)ref.W * ref.H;
+    return state_bytes(ref.W, ref.H, (int)r.srcs.size()) + hw * 20 + 4096;
+}
+
+int vsm_submit(vsm_ctx *c, const vsm_view_req *req, uint64_t *ticket)
+{
+    if (!c || !req || !ticket) return VSM_EINVAL;
+    if (req->n_src < 1 || req->n_src > VSM_MAX_SRC || !req->src_ids)
+        return fail(c, VSM_EINVAL, "n_src %d outside [1, %d]", req->n_src, VSM_MAX_SRC);
+    if (c->images.find(req->ref_id) == c->images.end())
+        return fail(c, VSM_ENOENT, "unknown reference image %d", req->ref_id);
+    for (int s = 0; s < req->n_src; ++s)
+        if (c->images.find(req->src_ids[s]) == c->images.end())
+            return fail(c, VSM_ENOENT, "unknown source image %d", req->src_ids[s]);
+    std::lock_guard<std::mutex> lk(c->serve_mu);
+    auto r = std::make_unique<ServeReq>();
+    r->ticket = c->next_ticket++;
+    r->ref = req->ref_id;
+    r->srcs.assign(req->src_ids, req->src_ids + req->n_src);
+    r->dmin = req->depth_min;
+    r->dmax = req->depth_max;
+    r->footprint = serve_footprint(c, *r);
+    if (c->budget && r->footprint > c->budget)
+        return fail(c, VSM_ENOMEM, "request footprint %zu exceeds the HBM budget", r->footprint);
+    *ticket = r->ticket;
+    c->queue.push_back(std::move(r));
+    return VSM_OK;
+}
+
+static int serve_admit(vsm_ctx *c)
+{
+    while (!c->queue.empty() && (int)c->active.size() < kServeMaxViews) {
+        ServeReq &r = *c->queue.front();
+        if (c->budget && c->in_use + r.footprint > c->budget) break;   // wait for memory
+        vsm_view_req q;
+        memset(&q, 0, sizeof q);
+        q.ref_id = r.ref;
+        q.n_src = (int)r.srcs.size();
+        q.src_ids = r.srcs.data();
+        q.depth_min = r.dmin;
+        q.depth_max = r.dmax;
+        int rc = prepare_view(c, q, 0, (uint32_t)r.ref, r.vs);
+        std::unique_ptr<ServeReq> owned = std::move(c->queue.front());
+        c->queue.pop_front();
+        if (rc == VSM_ENOMEM) {           // budget raced: requeue at the front
+            c->queue.push_front(std::move(owned));
+            break;
+        }
+        if (rc) {
+            owned->done = true;
+            owned->error = rc;
+            owned->errmsg = c->err;
+            c->reqs[owned->ticket] = std::move(owned);
+            continue;
+        }
+        const size_t hw = (size_t)r.vs.desc.W * r.vs.desc.H;
+        rc = dalloc(c, &r.out_depth, hw * 5);
+        if (rc) {
+            release_view(c, r.vs);
+            c->queue.push_front(std::move(owned));
+            break;
+        }
+        r.out_normal = r.out_depth + hw;
+        r.out_cost = r.out_depth + 4 * hw;
+        r.host_step = 0;
+        c->active.push_back(owned.get());
+        c->reqs[owned->ticket] = std::move(owned);
+        // random initial hypotheses for the new view
+        ViewDesc *dv = nullptr;
+        Tile *dt = nullptr;
+        std::vector<Tile> tl;
+        tiles_for(r.vs.desc, 0, tl);
+        if ((rc = dalloc(c, &dv, 1)) || (rc = dalloc(c, &dt, tl.size()))) { dfree(c, dv); return rc; }
+        HIPCHK(c, hipMemcpyAsync(dv, &r.vs.desc, sizeof(ViewDesc), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(dt, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, launch_init(dv, dt, (int)tl.size(), make_const(c->params), 1, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        dfree(c, dv);
+        dfree(c, dt);
+    }
+    return VSM_OK;
+}
+
+int vsm_serve_step(vsm_ctx *c, int32_t *n_active, int32_t *n_queued)
+{
+    if (!c) return VSM_EINVAL;
+    std::lock_guard<std::mutex> lk(c->serve_mu);
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = serve_admit(c);
+    if (rc) return rc;
+    const int na = (int)c->active.size();
+    if (na > 0) {
+        if (!c->serve_views) {
+            if ((rc = dalloc(c, &c->serve_views, kServeMaxViews))) return rc;
+        }
+        std::vector<ViewDesc> hv(na);
+        std::vector<Tile> tiles;
+        int maxs = 1;
+        for (int v = 0; v < na; ++v) {
+            ServeReq &r = *c->active[v];
+            hv[v] = r.vs.desc;
+            hv[v].step = r.host_step;
+            maxs = std::max(maxs, hv[v].n_src);
+            tiles_for(hv[v], v, tiles);
+        }
+        if ((int)tiles.size() > c->serve_cap_tiles) {
+            dfree(c, c->serve_tiles);
+            c->serve_tiles = nullptr;
+            c->serve_cap_tiles = 0;
+            if ((rc = dalloc(c, &c->serve_tiles, tiles.size() * 2))) return rc;
+            c->serve_cap_tiles = (int)tiles.size() * 2;
+        }
+        HIPCHK(c, hipMemcpyAsync(c->serve_views, hv.data(), na * sizeof(ViewDesc), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->serve_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream));
+        if ((rc = record_halfstep(c, maxs, c->serve_views, c->serve_tiles, (int)tiles.size(),
+                                  make_const(c->params), c->timing)))
+            return rc;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        // retire finished views
+        std::vector<ServeReq *> still;
+        for (int v = 0; v < na; ++v) {
+            ServeReq &r = *c->active[v];
+            r.host_step += 1;
+            if (r.host_step < r.vs.desc.num_steps) { still.push_back(&r); continue; }
+            r.vs.desc.step = r.host_step;
+            const size_t hw = (size_t)r.vs.desc.W * r.vs.desc.H;
+            const PMConst pc = make_const(c->params);
+            ViewDesc *dv = c->serve_views + v;
+            HIPCHK(c, hipMemcpyAsync(dv, &r.vs.desc, sizeof(ViewDesc), hipMemcpyHostToDevice, c->stream));
+            if (c->params.filter) {
+                std::vector<Tile> tl;
+                tiles_for(r.vs.desc, 0, tl);
+                Tile *dt = nullptr;
+                if ((rc = dalloc(c, &dt, tl.size()))) return rc;
+                HIPCHK(c, hipMemcpyAsync(dt, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream));
+                HIPCHK(c, launch_filter(r.vs.desc.n_src, dv, dt, (int)tl.size(), pc, nullptr, c->stream));
+                HIPCHK(c, hipStreamSynchronize(c->stream));
+                dfree(c, dt);
+            }
+            HIPCHK(c, launch_finalize(dv, 0, (int)hw, r.out_depth, r.out_normal, r.out_cost, nullptr, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            release_view(c, r.vs);
+            r.done = true;
+        }
+        c->active.swap(still);
+        rc = read_counters(c);
+        if (rc) return rc;
+    }
+    if (n_active) *n_active = (int)c->active.size();
+    if (n_queued) *n_queued = (int)c->queue.size();
+    return VSM_OK;
+}
+
+int vsm_poll(vsm_ctx *c, uint64_t ticket, vsm_view_out *out, int32_t *done)
+{
+    if (!c || !done) return VSM_EINVAL;
+    std::lock_guard<std::mutex> lk(c->serve_mu);
+    auto it = c->reqs.find(ticket);
+    if (it == c->reqs.end()) {
+        for (auto &q : c->queue)
+            if (q->ticket == ticket) { *done = 0; return VSM_OK; }
+        return fail(c, VSM_ENOENT, "unknown ticket %llu", (unsigned long long)ticket);
+    }
+    ServeReq &r = *it->second;
+    if (!r.done) { *done = 0; return VSM_OK; }
+    *done = 1;
+    if (r.error) {
+        c->err = r.errmsg;
+        int e = r.error;
+        c->reqs.erase(it);
+        return e;
+    }
+    const ImageRec &ref = c->images[r.ref];
+    const size_t hw = (size_t)ref.W * ref.H;
+    if (out && out->depth) HIPCHK(c, hipMemcpyAsync(out->depth, r.out_depth, hw * 4, hipMemcpyDeviceToHost, c->stream));
+    if (out && out->normal) HIPCHK(c, hipMemcpyAsync(out->normal, r.out_normal, hw * 12, hipMemcpyDeviceToHost, c->stream));
+    if (out && out->cost) HIPCHK(c, hipMemcpyAsync(out->cost, r.out_cost, hw * 4, hipMemcpyDeviceToHost, c->stream));
+    if (out && out->consistency) memset(out->consistency, 0, hw * 4);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(c, r.out_depth);
+    c->reqs.erase(it);
+    return VSM_OK;
+}
+
+int vsm_get_stats(vsm_ctx *c, vsm_stats *o)
+{
+    if (!c || !o) return VSM_EINVAL;
+    memset(o, 0, sizeof *o);
+    o->halfstep_ms = c->halfstep_ms;
+    o->halfstep_launches = c->halfstep_launches;
+    o->ncc_evals = c->acc_ncc;
+    o->geo_evals = c->acc_geo;
+    o->taps = c->acc_taps;
+    o->fuse_ms = c->fuse_ms;
+    o->fuse_launches = c->fuse_launches;
+    o->fuse_pixels = c->fuse_pixels;
+    o->hbm_bytes_in_use = c->in_use;
+    o->hbm_bytes_peak = c->peak;
+    return VSM_OK;
+}
+
+int vsm_reset_stats(vsm_ctx *c)
+{
+    if (!c) return VSM_EINVAL;
+    c->halfstep_ms = 0.0;
+    c->halfstep_launches = 0;
+    c->acc_ncc = c->acc_geo = c->acc_taps = 0;
+    c->fuse_ms = 0.0;
+    c->fuse_launches = c->fuse_pixels = 0;
+    c->peak = c->in_use;
+    return VSM_OK;
+}
+
+int vsm_set_timing(vsm_ctx *c, int32_t enabled)
+{
+    if (!c) return VSM_EINVAL;
+    c->timing = enabled != 0;
+    return VSM_OK;
+}
+
+int vsm_synchronize(vsm_ctx *c)
+{
+    if (!c) return VSM_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VSM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,298 @@
+// vsm_fuse.hip -- 2.5D DSM fusion kernels (HBM-bound, gfx950).
+//
+// Per view (reference aggregate_2p5d_util.py:45-106, lib/proj_to_grid.py:41-81,
+// produce_dsm.py:46-58):
+//   back-projection X = inv(P4) (col, row, 1, d), divide by X3, in FP64 with a
+//   fixed operation order (compiled -ffp-contract=off) so grid indices are
+//   bit-exact with the CPU restatement; row = floor((ul_n - N)/e_res),
+//   col = floor((E - ul_e)/n_res) (the reference's xresolution/yresolution
+//   pairing, lib/proj_to_grid.py:42-43); per-cell nanmax through a 64-bit
+//   atomicMax on an order-preserving encoding of the FP64 height (max is
+//   order-independent, so the raster does not depend on atomic ordering);
+//   NaN cells filled with the median of the in-bounds non-NaN 3x3 neighbours
+//   of the unfilled raster (lib/proj_to_grid.py:64-79), cast to float32;
+//   3x3 median with replicated border (produce_dsm.py:58).
+// Across views (aggregate_2p5d.py:66-81): cells with fewer than
+// min_measurements values -> NaN, median, MAD = median |x - med|, drop
+// |x - med| > MAD, mean (FP64 accumulation), then the 3x3 median.
+// Median rule (NaN-aware): NaN excluded; even count -> mean of the two middle
+// values; no values -> NaN.
+#include "vsm_device.h"
+#include "vsm_kernels.h"
+
+namespace vsm {
+
+__device__ __forceinline__ unsigned long long enc_f64(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double dec_f64(unsigned long long k)
+{
+    const unsigned long long b = (k & 0x8000000000000000ull) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+    return __longlong_as_double((long long)b);
+}
+
+__device__ __forceinline__ void cell_of(const DsmGridDev &g, double E, double N, int &row, int &col,
+                                        bool &ok)
+{
+    const double fr = floor((g.ul_n - N) / g.e_res);
+    const double fc = floor((E - g.ul_e) / g.n_res);
+    ok = fr >= 0.0 && fc >= 0.0 && fr < (double)g.n_size && fc < (double)g.e_size;
+    row = ok ? (int)fr : 0;
+    col = ok ? (int)fc : 0;
+}
+
+// One thread per depth pixel; d <= 0 or non-finite -> no point
+// (aggregate_2p5d_util.py:76, 92-93).
+__global__ void k_rasterize_view(const float *__restrict__ depth, int W, int H,
+                                 const double *__restrict__ Mi, DsmGridDev g,
+                                 unsigned long long *__restrict__ cell_max)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= W * H) return;
+    const float d32 = depth[p];
+    if (!(d32 > 0.f) || !isfinite(d32)) return;
+    const double col = (double)(p % W), row = (double)(p / W), d = (double)d32;
+    double X[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        X[i] = ((Mi[4 * i + 0] * col + Mi[4 * i + 1] * row) + Mi[4 * i + 2]) + Mi[4 * i + 3] * d;
+    const double E = X[0] / X[3], N = X[1] / X[3], h = X[2] / X[3];
+    if (!isfinite(E) || !isfinite(N) || !isfinite(h)) return;
+    int r, c;
+    bool ok;
+    cell_of(g, E, N, r, c, ok);
+    if (!ok) return;
+    atomicMax(&cell_max[(size_t)r * g.e_size + c], enc_f64(h));
+}
+
+__global__ void k_rasterize_points(const double *__restrict__ pts, long long n, DsmGridDev g,
+                                   unsigned long long *__restrict__ cell_max)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double E = pts[3 * i], N = pts[3 * i + 1], h = pts[3 * i + 2];
+    if (isnan(h) || isnan(E) || isnan(N)) return;
+    int r, c;
+    bool ok;
+    cell_of(g, E, N, r, c, ok);
+    if (!ok) return;
+    atomicMax(&cell_max[(size_t)r * g.e_size + c], enc_f64(h));
+}
+
+// Small in-register sort (insertion) of up to 9 doubles/floats.
+template <typename T, int N>
+__device__ __forceinline__ void sort_small(T (&v)[N], int n)
+{
+#pragma unroll
+    for (int i = 1; i < N; ++i) {
+        if (i < n) {
+            T x = v[i];
+#pragma unroll
+            for (int j = i - 1; j >= 0; --j) {
+                if (v[j] > x) { v[j + 1] = v[j]; v[j] = x; }
+            }
+        }
+    }
+}
+
+template <typename T, int N>
+__device__ __forceinline__ T median_sorted(const T (&v)[N], int n)
+{
+    T lo = v[0], hi = v[0];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        if (i == (n - 1) / 2) lo = v[i];
+        if (i == n / 2) hi = v[i];
+    }
+    return (n & 1) ? hi : (T)0.5 * (lo + hi);
+}
+
+// Decode + hole fill.  Output float (per-view DSM, then medianBlur) or double
+// (proj_to_grid stand-alone).
+template <typename OUT>
+__global__ void k_decode_fill(const unsigned long long *__restrict__ cell_max, int e_size,
+                              int n_size, OUT *__restrict__ dsm)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)e_size * n_size) return;
+    const int r = (int)(i / e_size), c = (int)(i - (long long)r * e_size);
+    const unsigned long long k = cell_max[i];
+    if (k != 0ull) { dsm[i] = (OUT)dec_f64(k); return; }
+    double v[9];
+    int n = 0;
+#pragma unroll
+    for (int dr = -1; dr <= 1; ++dr)
+#pragma unroll
+        for (int dc = -1; dc <= 1; ++dc) {
+            const int rr = r + dr, cc = c + dc;
+            const int slot = (dr + 1) * 3 + (dc + 1);
+            unsigned long long kk = 0ull;
+            if (rr >= 0 && cc >= 0 && rr < n_size && cc < e_size) kk = cell_max[(size_t)rr * e_size + cc];
+            v[slot] = kk ? dec_f64(kk) : (double)INFINITY;
+            n += kk ? 1 : 0;
+        }
+    if (n == 0) { dsm[i] = (OUT)NAN; return; }
+    sort_small<double, 9>(v, 9);
+    dsm[i] = (OUT)median_sorted<double, 9>(v, n);
+}
+
+// 3x3 median, replicated border, NaN excluded, all-NaN -> NaN.
+__global__ void k_median3(const float *__restrict__ in, int e_size, int n_size,
+                          float *__restrict__ out)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)e_size * n_size) return;
+    const int r = (int)(i / e_size), c = (int)(i - (long long)r * e_size);
+    float v[9];
+    int n = 0;
+#pragma unroll
+    for (int dr = -1; dr <= 1; ++dr)
+#pragma unroll
+        for (int dc = -1; dc <= 1; ++dc) {
+            const int rr = min(max(r + dr, 0), n_size - 1), cc = min(max(c + dc, 0), e_size - 1);
+            const float x = in[(size_t)rr * e_size + cc];
+            const int slot = (dr + 1) * 3 + (dc + 1);
+            v[slot] = x == x ? x : INFINITY;
+            n += x == x ? 1 : 0;
+        }
+    if (n == 0) { out[i] = NAN; return; }
+    sort_small<float, 9>(v, 9);
+    out[i] = median_sorted<float, 9>(v, n);
+}
+
+// Bitonic sort network over N registers (static indices only).
+template <int N>
+__device__ __forceinline__ void bitonic(float (&v)[N])
+{
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const bool up = (i & k) == 0;
+                    const float a = v[i], b = v[l];
+                    const bool sw = up ? (a > b) : (a < b);
+                    v[i] = sw ? b : a;
+                    v[l] = sw ? a : b;
+                }
+            }
+}
+
+template <int N>
+__global__ void k_fuse_stack(const float *__restrict__ stack, int nviews, long long ncell,
+                             int min_measurements, float *__restrict__ out)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncell) return;
+    float v[N];
+    int n = 0;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        float x = k < nviews ? stack[(size_t)k * ncell + i] : NAN;
+        const bool ok = x == x;
+        n += ok ? 1 : 0;
+        v[k] = ok ? x : INFINITY;
+    }
+    if (n < min_measurements || n == 0) { out[i] = NAN; return; }
+    float s[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) s[k] = v[k];
+    bitonic<N>(s);
+    const float med = median_sorted<float, N>(s, n);
+#pragma unroll
+    for (int k = 0; k < N; ++k) s[k] = v[k] == INFINITY ? INFINITY : fabsf(v[k] - med);
+    bitonic<N>(s);
+    const float mad = median_sorted<float, N>(s, n);
+    double acc = 0.0;
+    int m = 0;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        if (v[k] != INFINITY && !(fabsf(v[k] - med) > mad)) { acc += (double)v[k]; ++m; }
+    }
+    out[i] = m > 0 ? (float)(acc / (double)m) : NAN;
+}
+
+__global__ void k_fill_u32(unsigned int *p, long long n, unsigned int v)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_u64n(unsigned long long *p, long long n, unsigned long long v)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+static inline unsigned int nblk(long long n, int b) { return (unsigned int)((n + b - 1) / b); }
+
+hipError_t launch_rasterize_view(const float *depth, int W, int H, const double *invP4_dev,
+                                 DsmGridDev g, unsigned long long *cell_max, hipStream_t st)
+{
+    k_rasterize_view<<<nblk((long long)W * H, 256), 256, 0, st>>>(depth, W, H, invP4_dev, g,
+                                                                  cell_max);
+    return hipGetLastError();
+}
+
+hipError_t launch_rasterize_points(const double *pts, long long n, DsmGridDev g,
+                                   unsigned long long *cell_max, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    k_rasterize_points<<<nblk(n, 256), 256, 0, st>>>(pts, n, g, cell_max);
+    return hipGetLastError();
+}
+
+hipError_t launch_decode_fill_f32(const unsigned long long *cell_max, int e_size, int n_size,
+                                  float *dsm, hipStream_t st)
+{
+    k_decode_fill<float><<<nblk((long long)e_size * n_size, 256), 256, 0, st>>>(cell_max, e_size,
+                                                                             n_size, dsm);
+    return hipGetLastError();
+}
+
+hipError_t launch_decode_fill_f64(const unsigned long long *cell_max, int e_size, int n_size,
+                                  double *dsm, hipStream_t st)
+{
+    k_decode_fill<double><<<nblk((long long)e_size * n_size, 256), 256, 0, st>>>(cell_max, e_size,
+                                                                              n_size, dsm);
+    return hipGetLastError();
+}
+
+hipError_t launch_median3(const float *in, int e_size, int n_size, float *out, hipStream_t st)
+{
+    k_median3<<<nblk((long long)e_size * n_size, 256), 256, 0, st>>>(in, e_size, n_size, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fuse_stack(const float *stack, int nviews, long long ncell, int min_measurements,
+                             float *out, hipStream_t st)
+{
+    const unsigned int b = nblk(ncell, 256);
+    if (nviews <= 8) k_fuse_stack<8><<<b, 256, 0, st>>>(stack, nviews, ncell, min_measurements, out);
+    else if (nviews <= 16) k_fuse_stack<16><<<b, 256, 0, st>>>(stack, nviews, ncell, min_measurements, out);
+    else if (nviews <= 32) k_fuse_stack<32><<<b, 256, 0, st>>>(stack, nviews, ncell, min_measurements, out);
+    else if (nviews <= 64) k_fuse_stack<64><<<b, 256, 0, st>>>(stack, nviews, ncell, min_measurements, out);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(unsigned int *p, long long n, unsigned int v, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    k_fill_u32<<<nblk(n, 256), 256, 0, st>>>(p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u64n(unsigned long long *p, long long n, unsigned long long v,
+                            hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    k_fill_u64n<<<nblk(n, 256), 256, 0, st>>>(p, n, v);
+    return hipGetLastError();
+}
+
+}  // namespace vsm

@@ -1,0 +1,767 @@
+// vsm_pm.hip -- PatchMatch kernels for gfx950 (wave64).
+//
+// Work decomposition: one workgroup (256 threads, 4 waves) per 32x16 tile of
+// one reference view; in a red-black half-step each thread owns one pixel of
+// the active colour.  A launch covers the tiles of every view in a batch (the
+// tile list maps blockIdx -> (view, tile)), so batched and one-at-a-time
+// solves run the same per-pixel code and give bit-identical results.
+//
+// Numerics: per tile, the FP64 matrices M_s (ref (x,y,1,d) -> src) are
+// re-centred on the tile centre and on an integer source anchor, then
+// normalised and rounded to FP32; all per-tap arithmetic is FP32 on small
+// coordinates (|x| ~ 1e2), which keeps source positions accurate to ~1e-5 px
+// at 600 km camera range.  Compiled with -ffp-contract=off: every fused
+// multiply-add is an explicit fmaf, so recomputed quantities are bit-identical.
+//
+// Semantics follow the oracle (oracle/pm_oracle.c): bilaterally weighted NCC,
+// adaptive red-black propagation (best stored cost among 2k+1 steps in each of
+// four directions), random and perturbed hypotheses, Monte-Carlo view
+// selection with triangulation/incident-angle priors, geometric
+// forward-backward term, filter.
+#include "vsm_device.h"
+#include "vsm_kernels.h"
+
+namespace vsm {
+
+struct SrcLds {
+    float A[12];                 // tile-centred ref -> src (anchor-relative), normalised
+    float B[12];                 // src (anchor-relative) -> ref (tile-centre-relative)
+    int ax, ay, W, H;
+    float lox, hix, loy, hiy;    // in-image box in anchor-relative coordinates
+    const uint32_t *quad;
+    const float *depth;
+    float ptri, tri, g0, g1, g2;
+};
+
+struct Proj {
+    float q0x, q0y, q0z;         // centre projection
+    float hux, huy, huz;         // d q / d dx
+    float hvx, hvy, hvz;         // d q / d dy
+};
+
+// FP64 tile re-centring of one source's matrices (run by one thread per source).
+__device__ void setup_source(const ViewDesc &V, int s, int uc, int vc, SrcLds &o)
+{
+    const double *M = V.M[s];
+    const double *Mb = V.Mb[s];
+    const double dmid = 0.5 * ((double)V.depth_min + (double)V.depth_max);
+    const double qx = M[0] * uc + M[1] * vc + M[2] + M[3] * dmid;
+    const double qy = M[4] * uc + M[5] * vc + M[6] + M[7] * dmid;
+    const double qz = M[8] * uc + M[9] * vc + M[10] + M[11] * dmid;
+    double px = qz > 0.0 ? qx / qz : 0.0, py = qz > 0.0 ? qy / qz : 0.0;
+    px = fmin(fmax(px, -1.0e7), 1.0e7);
+    py = fmin(fmax(py, -1.0e7), 1.0e7);
+    const int ax = (int)rint(px), ay = (int)rint(py);
+    double A[12], B[12];
+    for (int r = 0; r < 3; ++r) {
+        A[4 * r + 0] = M[4 * r + 0];
+        A[4 * r + 1] = M[4 * r + 1];
+        A[4 * r + 2] = M[4 * r + 0] * uc + M[4 * r + 1] * vc + M[4 * r + 2];
+        A[4 * r + 3] = M[4 * r + 3];
+        B[4 * r + 0] = Mb[4 * r + 0];
+        B[4 * r + 1] = Mb[4 * r + 1];
+        B[4 * r + 2] = Mb[4 * r + 0] * ax + Mb[4 * r + 1] * ay + Mb[4 * r + 2];
+        B[4 * r + 3] = Mb[4 * r + 3];
+    }
+    for (int k = 0; k < 4; ++k) {
+        A[k] -= ax * A[8 + k];
+        A[4 + k] -= ay * A[8 + k];
+        B[k] -= uc * B[8 + k];
+        B[4 + k] -= vc * B[8 + k];
+    }
+    double ma = 0.0, mb = 0.0;
+    for (int k = 0; k < 12; ++k) { ma = fmax(ma, fabs(A[k])); mb = fmax(mb, fabs(B[k])); }
+    ma = ma > 0.0 ? 1.0 / ma : 1.0;
+    mb = mb > 0.0 ? 1.0 / mb : 1.0;
+    for (int k = 0; k < 12; ++k) { o.A[k] = (float)(A[k] * ma); o.B[k] = (float)(B[k] * mb); }
+    o.ax = ax; o.ay = ay;
+    o.W = V.src_w[s]; o.H = V.src_h[s];
+    o.lox = (float)(-ax); o.hix = (float)(o.W - 1 - ax);
+    o.loy = (float)(-ay); o.hiy = (float)(o.H - 1 - ay);
+    o.quad = V.src_quad[s];
+    o.depth = V.src_depth[s];
+    o.ptri = V.ptri[s]; o.tri = V.tri[s];
+    o.g0 = V.g[s][0]; o.g1 = V.g[s][1]; o.g2 = V.g[s][2];
+}
+
+__device__ __forceinline__ Proj make_proj(const SrcLds &S, float du, float dv, float d, float a,
+                                          float b)
+{
+    Proj P;
+    P.q0x = fmaf(S.A[0], du, fmaf(S.A[1], dv, fmaf(S.A[3], d, S.A[2])));
+    P.q0y = fmaf(S.A[4], du, fmaf(S.A[5], dv, fmaf(S.A[7], d, S.A[6])));
+    P.q0z = fmaf(S.A[8], du, fmaf(S.A[9], dv, fmaf(S.A[11], d, S.A[10])));
+    P.hux = fmaf(a, S.A[3], S.A[0]);
+    P.huy = fmaf(a, S.A[7], S.A[4]);
+    P.huz = fmaf(a, S.A[11], S.A[8]);
+    P.hvx = fmaf(b, S.A[3], S.A[1]);
+    P.hvy = fmaf(b, S.A[7], S.A[5]);
+    P.hvz = fmaf(b, S.A[11], S.A[9]);
+    return P;
+}
+
+// Bilinear sample (DN) at anchor-relative (px, py), already clamped in-image.
+__device__ __forceinline__ float sample_quad(const SrcLds &S, float px, float py)
+{
+    const float fx0 = floorf(px), fy0 = floorf(py);
+    const float fx = px - fx0, fy = py - fy0;
+    const int ix = S.ax + (int)fx0, iy = S.ay + (int)fy0;
+    const uint32_t v = S.quad[iy * S.W + ix];
+    const float a = (float)(v & 0xFFu), b = (float)((v >> 8) & 0xFFu);
+    const float c = (float)((v >> 16) & 0xFFu), d = (float)(v >> 24);
+    const float top = fmaf(fx, b - a, a);
+    const float bot = fmaf(fx, d - c, c);
+    return fmaf(fy, bot - top, top);
+}
+
+__device__ __forceinline__ float tap_weight(const PMConst &c, float r, int dx, int dy)
+{
+    return exp2f(-fmaf(r * r, c.ccc, (float)(dx * dx + dy * dy) * c.css));
+}
+
+struct RefStats {
+    float rc, sw, mr, var_r;
+};
+
+__device__ __forceinline__ RefStats ref_stats(const PMConst &c, const float *s_ref, int lbase)
+{
+    RefStats st;
+    st.rc = s_ref[lbase];
+    float sw = 0.f, sr = 0.f, srr = 0.f;
+    for (int j = 0; j < c.ntaps; ++j) {
+        const int dy = -c.radius + j * c.step;
+        const float *row = s_ref + lbase + dy * kLdsW;
+        for (int i = 0; i < c.ntaps; ++i) {
+            const int dx = -c.radius + i * c.step;
+            const float r = row[dx] - st.rc;
+            const float w = tap_weight(c, r, dx, dy);
+            sw += w;
+            sr = fmaf(w, r, sr);
+            srr = fmaf(w * r, r, srr);
+        }
+    }
+    st.sw = sw;
+    st.mr = sr / sw;
+    st.var_r = srr / sw - st.mr * st.mr;
+    return st;
+}
+
+// 1 - bilaterally weighted NCC of the plane P against source S.
+__device__ float ncc_cost(const PMConst &c, const SrcLds &S, const Proj &P, const float *s_ref,
+                          int lbase, const RefStats &R)
+{
+    if (!(P.q0z > 0.f)) return kMaxCost;
+    const float cx = P.q0x / P.q0z, cy = P.q0y / P.q0z;
+    if (!(cx >= S.lox && cx <= S.hix && cy >= S.loy && cy <= S.hiy)) return kMaxCost;
+    const float sc = sample_quad(S, cx, cy);
+    float ss = 0.f, sss = 0.f, srs = 0.f;
+    const int rad = c.radius, st = c.step, nt = c.ntaps;
+    for (int j = 0; j < nt; ++j) {
+        const int dy = -rad + j * st;
+        const float fdy = (float)dy, frad = (float)(-rad);
+        const float rx = fmaf(fdy, P.hvx, fmaf(frad, P.hux, P.q0x));
+        const float ry = fmaf(fdy, P.hvy, fmaf(frad, P.huy, P.q0y));
+        const float rz = fmaf(fdy, P.hvz, fmaf(frad, P.huz, P.q0z));
+        const float *row = s_ref + lbase + dy * kLdsW;
+#pragma unroll 2
+        for (int i = 0; i < nt; ++i) {
+            const int dx = -rad + i * st;
+            const float fi = (float)(i * st);
+            const float qx = fmaf(fi, P.hux, rx);
+            const float qy = fmaf(fi, P.huy, ry);
+            const float qz = fmaf(fi, P.huz, rz);
+            const float iz = __builtin_amdgcn_rcpf(qz);
+            float px = qx * iz, py = qy * iz;
+            px = fminf(fmaxf(px, S.lox), S.hix);
+            py = fminf(fmaxf(py, S.loy), S.hiy);
+            const float sv = sample_quad(S, px, py) - sc;
+            const float r = row[dx] - R.rc;
+            const float w = tap_weight(c, r, dx, dy);
+            const float ws = w * sv;
+            ss += ws;
+            sss = fmaf(ws, sv, sss);
+            srs = fmaf(w * r, sv, srs);
+        }
+    }
+    const float ms = ss / R.sw;
+    const float var_s = sss / R.sw - ms * ms;
+    if (R.var_r < c.min_var || var_s < c.min_var) return kMaxCost;
+    const float cov = srs / R.sw - R.mr * ms;
+    const float ncc = cov / sqrtf(R.var_r * var_s);
+    return fminf(fmaxf(1.f - ncc, 0.f), kMaxCost);
+}
+
+// Forward-backward reprojection error (px) through the source depth map.
+__device__ float geo_error(const PMConst &c, const SrcLds &S, const Proj &P, float du, float dv)
+{
+    const float gmax = c.geom_max;
+    if (S.depth == nullptr) return gmax;
+    if (!(P.q0z > 0.f)) return gmax;
+    const float px = P.q0x / P.q0z, py = P.q0y / P.q0z;
+    if (!(px >= S.lox && px <= S.hix && py >= S.loy && py <= S.hiy)) return gmax;
+    const float fx0 = floorf(px), fy0 = floorf(py);
+    const int x0 = S.ax + (int)fx0, y0 = S.ay + (int)fy0;
+    const int x1 = min(x0 + 1, S.W - 1), y1 = min(y0 + 1, S.H - 1);
+    const float fx = px - fx0, fy = py - fy0;
+    const float da = S.depth[y0 * S.W + x0], db = S.depth[y0 * S.W + x1];
+    const float dc = S.depth[y1 * S.W + x0], de = S.depth[y1 * S.W + x1];
+    if (!(da > 0.f && db > 0.f && dc > 0.f && de > 0.f)) return gmax;
+    const float top = fmaf(fx, db - da, da), bot = fmaf(fx, de - dc, dc);
+    const float ds = fmaf(fy, bot - top, top);
+    const float bx = fmaf(S.B[0], px, fmaf(S.B[1], py, fmaf(S.B[3], ds, S.B[2])));
+    const float by = fmaf(S.B[4], px, fmaf(S.B[5], py, fmaf(S.B[7], ds, S.B[6])));
+    const float bz = fmaf(S.B[8], px, fmaf(S.B[9], py, fmaf(S.B[11], ds, S.B[10])));
+    if (!(bz > 0.f)) return gmax;
+    const float ex = bx / bz - du, ey = by / bz - dv;
+    return sqrtf(ex * ex + ey * ey);
+}
+
+__device__ __forceinline__ float total_cost(const PMConst &c, const SrcLds &S, const Proj &P,
+                                            float photo, int pass, float du, float dv)
+{
+    if (pass == 1) {
+        const float e = geo_error(c, S, P, du, dv);
+        return photo + c.geom_reg * fminf(e, c.geom_max);
+    }
+    return photo;
+}
+
+__device__ __forceinline__ void random_normal(const PMConst &c, float u1, float u2, float n[3])
+{
+    // sin from (1 - cos)(1 + cos): no cancellation near the cone axis
+    const float omc = u1 * c.one_minus_cos_max;
+    const float ct = 1.f - omc;
+    const float st = sqrtf(omc * (2.f - omc));
+    const float ph = 6.283185307179586f * u2;
+    float sp, cp;
+    sincosf(ph, &sp, &cp);
+    n[0] = st * cp;
+    n[1] = st * sp;
+    n[2] = ct;
+}
+
+__device__ __forceinline__ void perturb_normal(const PMConst &c, const float n[3], float amax,
+                                               float u1, float u2, float out[3])
+{
+    float h0 = 1.f, h1 = 0.f;
+    if (fabsf(n[0]) >= 0.9f) { h0 = 0.f; h1 = 1.f; }
+    // e1 = n x h, h = (h0, h1, 0)
+    float e1x = n[1] * 0.f - n[2] * h1, e1y = n[2] * h0 - n[0] * 0.f, e1z = n[0] * h1 - n[1] * h0;
+    const float l1 = sqrtf(e1x * e1x + e1y * e1y + e1z * e1z);
+    e1x /= l1; e1y /= l1; e1z /= l1;
+    const float e2x = n[1] * e1z - n[2] * e1y, e2y = n[2] * e1x - n[0] * e1z,
+                e2z = n[0] * e1y - n[1] * e1x;
+    const float al = amax * u1, ph = 6.283185307179586f * u2;
+    float sa, ca, sp, cp;
+    sincosf(al, &sa, &ca);
+    sincosf(ph, &sp, &cp);
+    float m0 = ca * n[0] + sa * (cp * e1x + sp * e2x);
+    float m1 = ca * n[1] + sa * (cp * e1y + sp * e2y);
+    float m2 = ca * n[2] + sa * (cp * e1z + sp * e2z);
+    const float lm = sqrtf(m0 * m0 + m1 * m1 + m2 * m2);
+    m0 /= lm; m1 /= lm; m2 /= lm;
+    if (m2 < c.cos_max_normal) { out[0] = n[0]; out[1] = n[1]; out[2] = n[2]; }
+    else { out[0] = m0; out[1] = m1; out[2] = m2; }
+}
+
+// Sampling weight of source S for the current hypothesis (alpha * priors).
+__device__ __forceinline__ float view_weight(const PMConst &c, const SrcLds &S, float alpha,
+                                             const float n[3], float jtn)
+{
+    float ci = fabsf(n[0] * S.g0 + n[1] * S.g1 + n[2] * S.g2) / jtn;
+    ci = fminf(ci, 1.f);
+    const float inc = acosf(ci);
+    return alpha * S.ptri * expf(-inc * inc * c.inv2i);
+}
+
+template <typename T>
+__device__ __forceinline__ T pick8(int k, T v0, T v1, T v2, T v3, T v4, T v5, T v6, T v7)
+{
+    T r = v0;
+    r = k == 1 ? v1 : r;
+    r = k == 2 ? v2 : r;
+    r = k == 3 ? v3 : r;
+    r = k == 4 ? v4 : r;
+    r = k == 5 ? v5 : r;
+    r = k == 6 ? v6 : r;
+    r = k == 7 ? v7 : r;
+    return r;
+}
+
+// Common tile prologue: reference tile (replicated border) into LDS and the
+// tile-centred source matrices.
+template <int MAXS>
+__device__ __forceinline__ void tile_prologue(const ViewDesc &V, const Tile &t, float *s_ref,
+                                              SrcLds *s_src)
+{
+    const int W = V.W, H = V.H;
+    for (int k = threadIdx.x; k < kLdsH * kLdsW; k += kThreads) {
+        const int ly = k / kLdsW, lx = k - ly * kLdsW;
+        const int gx = min(max(t.x0 + lx - kMaxRadius, 0), W - 1);
+        const int gy = min(max(t.y0 + ly - kMaxRadius, 0), H - 1);
+        s_ref[k] = (float)V.ref[gy * W + gx];
+    }
+    if ((int)threadIdx.x < V.n_src)
+        setup_source(V, threadIdx.x, t.x0 + kTileW / 2, t.y0 + kTileH / 2, s_src[threadIdx.x]);
+}
+
+template <int MAXS>
+__global__ __launch_bounds__(kThreads) void k_halfstep(const ViewDesc *__restrict__ views,
+                                                       const Tile *__restrict__ tiles, int ntiles,
+                                                       PMConst c, Counters *counters)
+{
+    __shared__ float s_ref[kLdsH * kLdsW];
+    __shared__ SrcLds s_src[MAXS];
+    __shared__ float s_c0[MAXS * kThreads];
+    __shared__ uint16_t s_list[MAXS * kThreads];
+    __shared__ unsigned int s_evals[2];
+
+    const Tile t = tiles[blockIdx.x];
+    const ViewDesc &V = views[t.view];
+    if (!V.active || V.step >= V.num_steps) return;       // uniform per workgroup
+    const int iter = V.step >> 1, color = V.step & 1, pass = V.pass;
+    const int W = V.W, H = V.H, S = V.n_src;
+    const int tid = threadIdx.x;
+    if (tid < 2) s_evals[tid] = 0u;
+    tile_prologue<MAXS>(V, t, s_ref, s_src);
+    __syncthreads();
+
+    const int ty = tid >> 4;
+    const int tx = ((tid & 15) << 1) + ((t.y0 + ty + color) & 1);
+    const int x = t.x0 + tx, y = t.y0 + ty;
+    unsigned int my_ncc = 0, my_geo = 0;
+    if (x < W && y < H) {
+        const size_t HW = (size_t)W * H;
+        const int p = y * W + x;
+        const float du = (float)(tx - kTileW / 2), dv = (float)(ty - kTileH / 2);
+        const int lbase = (ty + kMaxRadius) * kLdsW + tx + kMaxRadius;
+
+        // ---- candidate hypotheses ---------------------------------------
+        float hd[kNHyp], hn0[kNHyp], hn1[kNHyp], hn2[kNHyp];
+        const float4 h0 = V.hyp[p];
+        hd[0] = h0.x; hn0[0] = h0.y; hn1[0] = h0.z; hn2[0] = h0.w;
+        const int DX[4] = {-1, 1, 0, 0}, DY[4] = {0, 0, -1, 1};
+#pragma unroll
+        for (int dir = 0; dir < 4; ++dir) {
+            int bx = -1, by = -1;
+            float bc = 0.f;
+            for (int k = 0; k < c.n_far; ++k) {
+                const int dist = 2 * k + 1;
+                const int qx = x + DX[dir] * dist, qy = y + DY[dir] * dist;
+                if (qx < 0 || qy < 0 || qx >= W || qy >= H) break;
+                const float cq = V.cost[qy * W + qx];
+                if (bx < 0 || cq < bc) { bx = qx; by = qy; bc = cq; }
+            }
+            if (bx < 0) {
+                hd[1 + dir] = hd[0]; hn0[1 + dir] = hn0[0]; hn1[1 + dir] = hn1[0]; hn2[1 + dir] = hn2[0];
+            } else {
+                const float4 hq = V.hyp[by * W + bx];
+                hd[1 + dir] = hq.x - (hq.y * (float)(x - bx) + hq.z * (float)(y - by)) / hq.w;
+                hn0[1 + dir] = hq.y; hn1[1 + dir] = hq.z; hn2[1 + dir] = hq.w;
+            }
+        }
+        const uint32_t code = step_code(pass, iter, color);
+        float u0[4], u1[4];
+        rng_block(c, V.view_id, (uint32_t)p, code, 0u, u0);
+        rng_block(c, V.view_id, (uint32_t)p, code, 1u, u1);
+        const float range = V.depth_max - V.depth_min;
+        const float decay = powf(c.decay_base, (float)iter);
+        {
+            float n[3];
+            hd[5] = V.depth_min + u0[0] * range;
+            random_normal(c, u0[1], u0[2], n);
+            hn0[5] = n[0]; hn1[5] = n[1]; hn2[5] = n[2];
+            float d6 = hd[0] + (2.f * u0[3] - 1.f) * c.perturbation * range * decay;
+            hd[6] = fminf(fmaxf(d6, V.depth_min), V.depth_max);
+            hn0[6] = hn0[0]; hn1[6] = hn1[0]; hn2[6] = hn2[0];
+            const float nn[3] = {hn0[0], hn1[0], hn2[0]};
+            hd[7] = hd[0];
+            perturb_normal(c, nn, c.normal_perturbation * decay, u1[0], u1[1], n);
+            hn0[7] = n[0]; hn1[7] = n[1]; hn2[7] = n[2];
+        }
+
+        // ---- view selection on the current hypothesis --------------------
+        const RefStats R = ref_stats(c, s_ref, lbase);
+        const float a0 = -hn0[0] / hn2[0], b0 = -hn1[0] / hn2[0];
+        const float nh0[3] = {hn0[0], hn1[0], hn2[0]};
+        float jtn;
+        {
+            const float *J = V.J;
+            const float j0 = J[0] * nh0[0] + J[3] * nh0[1] + J[6] * nh0[2];
+            const float j1 = J[1] * nh0[0] + J[4] * nh0[1] + J[7] * nh0[2];
+            const float j2 = J[2] * nh0[0] + J[5] * nh0[1] + J[8] * nh0[2];
+            jtn = sqrtf(j0 * j0 + j1 * j1 + j2 * j2);
+        }
+        float tot = 0.f;
+        int lastpos = 0;
+        for (int s = 0; s < S; ++s) {
+            const SrcLds &Ss = s_src[s];
+            const Proj P = make_proj(Ss, du, dv, hd[0], a0, b0);
+            const float c0 = ncc_cost(c, Ss, P, s_ref, lbase, R);
+            s_c0[s * kThreads + tid] = c0;
+            float prev = 0.f;
+            int np = 0;
+#pragma unroll
+            for (int dir = 0; dir < 4; ++dir) {
+                const int qx = x + DX[dir], qy = y + DY[dir];
+                if (qx < 0 || qy < 0 || qx >= W || qy >= H) continue;
+                prev += V.prob[(size_t)s * HW + qy * W + qx];
+                ++np;
+            }
+            prev = np ? prev / (float)np : 0.5f;
+            const float L = expf(-c0 * c0 * c.inv2n);
+            const float a1 = L * (prev * 0.999f + (1.f - prev) * 0.001f);
+            const float a0v = 0.5f * ((1.f - prev) * 0.999f + prev * 0.001f);
+            const float alpha = a1 / (a1 + a0v);
+            V.prob[(size_t)s * HW + p] = alpha;
+            const float q = view_weight(c, Ss, alpha, nh0, jtn);
+            tot += q;
+            if (q > 0.f) lastpos = s;
+        }
+        my_ncc += S;
+
+        // ---- Monte-Carlo draw of sources ---------------------------------
+        int nlist = 0;
+        float wsum = 0.f, acc0 = 0.f;
+        if (!(tot > 0.f)) {
+            for (int s = 0; s < S; ++s) {
+                const SrcLds &Ss = s_src[s];
+                float cc = s_c0[s * kThreads + tid];
+                if (pass == 1) {
+                    const Proj P = make_proj(Ss, du, dv, hd[0], a0, b0);
+                    cc = total_cost(c, Ss, P, cc, pass, du, dv);
+                    ++my_geo;
+                }
+                acc0 += 1.f * cc;
+                wsum += 1.f;
+                s_list[nlist * kThreads + tid] = (uint16_t)(s | (1 << 8));
+                ++nlist;
+            }
+        } else {
+            float tk[kMaxSamples];
+#pragma unroll
+            for (int b = 0; b < kMaxSamples / 4; ++b) {
+                float u[4];
+                rng_block(c, V.view_id, (uint32_t)p, code, 2u + (uint32_t)b, u);
+                tk[4 * b + 0] = u[0] * tot; tk[4 * b + 1] = u[1] * tot;
+                tk[4 * b + 2] = u[2] * tot; tk[4 * b + 3] = u[3] * tot;
+            }
+            float acc = 0.f;
+            for (int s = 0; s < S; ++s) {
+                const SrcLds &Ss = s_src[s];
+                const float alpha = V.prob[(size_t)s * HW + p];
+                const float q = view_weight(c, Ss, alpha, nh0, jtn);
+                const float lo = acc;
+                acc = acc + q;
+                const float hi = s == lastpos ? INFINITY : acc;
+                int cnt = 0;
+#pragma unroll
+                for (int k = 0; k < kMaxSamples; ++k)
+                    cnt += (k < c.num_samples && tk[k] >= lo && tk[k] < hi) ? 1 : 0;
+                if (cnt > 0) {
+                    float cc = s_c0[s * kThreads + tid];
+                    if (pass == 1) {
+                        const Proj P = make_proj(Ss, du, dv, hd[0], a0, b0);
+                        cc = total_cost(c, Ss, P, cc, pass, du, dv);
+                        ++my_geo;
+                    }
+                    acc0 += (float)cnt * cc;
+                    wsum += (float)cnt;
+                    s_list[nlist * kThreads + tid] = (uint16_t)(s | (cnt << 8));
+                    ++nlist;
+                }
+            }
+        }
+
+        // ---- evaluate the other seven hypotheses -------------------------
+        float best_c = acc0 / wsum;
+        int best = 0;
+        for (int k = 1; k < kNHyp; ++k) {
+            const float dk = pick8(k, hd[0], hd[1], hd[2], hd[3], hd[4], hd[5], hd[6], hd[7]);
+            const float n0 = pick8(k, hn0[0], hn0[1], hn0[2], hn0[3], hn0[4], hn0[5], hn0[6], hn0[7]);
+            const float n1 = pick8(k, hn1[0], hn1[1], hn1[2], hn1[3], hn1[4], hn1[5], hn1[6], hn1[7]);
+            const float n2 = pick8(k, hn2[0], hn2[1], hn2[2], hn2[3], hn2[4], hn2[5], hn2[6], hn2[7]);
+            const float ak = -n0 / n2, bk = -n1 / n2;
+            float acc = 0.f;
+            for (int j = 0; j < nlist; ++j) {
+                const uint32_t e = s_list[j * kThreads + tid];
+                const SrcLds &Ss = s_src[e & 0xFFu];
+                const Proj P = make_proj(Ss, du, dv, dk, ak, bk);
+                const float ph = ncc_cost(c, Ss, P, s_ref, lbase, R);
+                acc += (float)(e >> 8) * total_cost(c, Ss, P, ph, pass, du, dv);
+            }
+            my_ncc += nlist;
+            if (pass == 1) my_geo += nlist;
+            const float ck = acc / wsum;
+            if (ck < best_c) { best_c = ck; best = k; }
+        }
+        const float bd = pick8(best, hd[0], hd[1], hd[2], hd[3], hd[4], hd[5], hd[6], hd[7]);
+        const float b0n = pick8(best, hn0[0], hn0[1], hn0[2], hn0[3], hn0[4], hn0[5], hn0[6], hn0[7]);
+        const float b1n = pick8(best, hn1[0], hn1[1], hn1[2], hn1[3], hn1[4], hn1[5], hn1[6], hn1[7]);
+        const float b2n = pick8(best, hn2[0], hn2[1], hn2[2], hn2[3], hn2[4], hn2[5], hn2[6], hn2[7]);
+        V.hyp[p] = make_float4(bd, b0n, b1n, b2n);
+        V.cost[p] = best_c;
+    }
+    if (counters) {
+        atomicAdd(&s_evals[0], my_ncc);
+        atomicAdd(&s_evals[1], my_geo);
+        __syncthreads();
+        if (tid == 0) {
+            atomicAdd(&counters->ncc_evals, (unsigned long long)s_evals[0]);
+            atomicAdd(&counters->geo_evals, (unsigned long long)s_evals[1]);
+            atomicAdd(&counters->tiles, 1ull);
+        }
+    }
+}
+
+// Random initialisation: uniform depth in the range, normal in the cone.
+__global__ __launch_bounds__(kThreads) void k_init(const ViewDesc *__restrict__ views,
+                                                   const Tile *__restrict__ tiles, PMConst c,
+                                                   int random_hyp)
+{
+    const Tile t = tiles[blockIdx.x];
+    const ViewDesc &V = views[t.view];
+    const size_t HW = (size_t)V.W * V.H;
+    for (int k = threadIdx.x; k < kTileW * kTileH; k += kThreads) {
+        const int x = t.x0 + (k & (kTileW - 1)), y = t.y0 + k / kTileW;
+        if (x >= V.W || y >= V.H) continue;
+        const int p = y * V.W + x;
+        if (random_hyp) {
+            float u[4], n[3];
+            rng_block(c, V.view_id, (uint32_t)p, init_code(V.pass), 0u, u);
+            random_normal(c, u[1], u[2], n);
+            V.hyp[p] = make_float4(V.depth_min + u[0] * (V.depth_max - V.depth_min), n[0], n[1], n[2]);
+        }
+        V.cost[p] = kMaxCost;
+        for (int s = 0; s < V.n_src; ++s) V.prob[(size_t)s * HW + p] = 0.5f;
+    }
+}
+
+// Cost of the stored hypotheses against every source: out[s*HW + p].
+template <int MAXS>
+__global__ __launch_bounds__(kThreads) void k_eval_cost(const ViewDesc *__restrict__ views,
+                                                        const Tile *__restrict__ tiles, PMConst c,
+                                                        float *out)
+{
+    __shared__ float s_ref[kLdsH * kLdsW];
+    __shared__ SrcLds s_src[MAXS];
+    const Tile t = tiles[blockIdx.x];
+    const ViewDesc &V = views[t.view];
+    tile_prologue<MAXS>(V, t, s_ref, s_src);
+    __syncthreads();
+    const size_t HW = (size_t)V.W * V.H;
+    for (int k = threadIdx.x; k < kTileW * kTileH; k += kThreads) {
+        const int tx = k & (kTileW - 1), ty = k / kTileW;
+        const int x = t.x0 + tx, y = t.y0 + ty;
+        if (x >= V.W || y >= V.H) continue;
+        const int p = y * V.W + x;
+        const int lbase = (ty + kMaxRadius) * kLdsW + tx + kMaxRadius;
+        const float du = (float)(tx - kTileW / 2), dv = (float)(ty - kTileH / 2);
+        const RefStats R = ref_stats(c, s_ref, lbase);
+        const float4 h = V.hyp[p];
+        const float a = -h.y / h.w, b = -h.z / h.w;
+        for (int s = 0; s < V.n_src; ++s) {
+            const SrcLds &Ss = s_src[s];
+            const Proj P = make_proj(Ss, du, dv, h.x, a, b);
+            const float ph = ncc_cost(c, Ss, P, s_ref, lbase, R);
+            out[(size_t)s * HW + p] = total_cost(c, Ss, P, ph, V.pass, du, dv);
+        }
+    }
+}
+
+// Filter: count consistent sources; zero the hypothesis when too few.
+template <int MAXS>
+__global__ __launch_bounds__(kThreads) void k_filter(const ViewDesc *__restrict__ views,
+                                                     const Tile *__restrict__ tiles, PMConst c,
+                                                     uint32_t *mask_out)
+{
+    __shared__ float s_ref[kLdsH * kLdsW];
+    __shared__ SrcLds s_src[MAXS];
+    const Tile t = tiles[blockIdx.x];
+    const ViewDesc &V = views[t.view];
+    tile_prologue<MAXS>(V, t, s_ref, s_src);
+    __syncthreads();
+    for (int k = threadIdx.x; k < kTileW * kTileH; k += kThreads) {
+        const int tx = k & (kTileW - 1), ty = k / kTileW;
+        const int x = t.x0 + tx, y = t.y0 + ty;
+        if (x >= V.W || y >= V.H) continue;
+        const int p = y * V.W + x;
+        const float4 h = V.hyp[p];
+        uint32_t m = 0u;
+        int count = 0;
+        if (!(h.x > 0.f)) {
+            if (mask_out) mask_out[p] = 0u;
+            continue;
+        }
+        {
+            const int lbase = (ty + kMaxRadius) * kLdsW + tx + kMaxRadius;
+            const float du = (float)(tx - kTileW / 2), dv = (float)(ty - kTileH / 2);
+            const RefStats R = ref_stats(c, s_ref, lbase);
+            const float a = -h.y / h.w, b = -h.z / h.w;
+            for (int s = 0; s < V.n_src; ++s) {
+                const SrcLds &Ss = s_src[s];
+                const Proj P = make_proj(Ss, du, dv, h.x, a, b);
+                const float ph = ncc_cost(c, Ss, P, s_ref, lbase, R);
+                if (!(1.f - ph >= c.f_min_ncc)) continue;
+                if (!(Ss.tri >= c.f_min_tri)) continue;
+                if (V.pass == 1 && !(geo_error(c, Ss, P, du, dv) <= c.f_geom_max)) continue;
+                m |= 1u << s;
+                ++count;
+            }
+        }
+        if (mask_out) mask_out[p] = m;
+        if (count < c.f_min_consistent) V.hyp[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+// Outputs: depth, camera-frame normal (facing the camera), cost.
+__global__ void k_finalize(const ViewDesc *__restrict__ views, int view, float *depth,
+                           float *normal, float *cost_out, float *depth_keep)
+{
+    const ViewDesc &V = views[view];
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= V.W * V.H) return;
+    const float4 h = V.hyp[p];
+    if (depth) depth[p] = h.x;
+    if (depth_keep) depth_keep[p] = h.x;
+    if (cost_out) cost_out[p] = V.cost[p];
+    if (normal) {
+        const float *N = V.N_out;
+        float n0 = N[0] * h.y + N[1] * h.z + N[2] * h.w;
+        float n1 = N[3] * h.y + N[4] * h.z + N[5] * h.w;
+        float n2 = N[6] * h.y + N[7] * h.z + N[8] * h.w;
+        const float l = sqrtf(n0 * n0 + n1 * n1 + n2 * n2);
+        if (l > 0.f) { n0 /= l; n1 /= l; n2 /= l; }
+        if (n2 > 0.f) { n0 = -n0; n1 = -n1; n2 = -n2; }
+        normal[3 * p + 0] = n0; normal[3 * p + 1] = n1; normal[3 * p + 2] = n2;
+    }
+}
+
+// Hypotheses from depth + camera-frame normals (resume from workspace files).
+__global__ void k_load_hyp(const ViewDesc *__restrict__ views, int view, const float *depth,
+                           const float *normal)
+{
+    const ViewDesc &V = views[view];
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= V.W * V.H) return;
+    float d = depth[p];
+    const float *N = V.N_in;
+    const float c0 = normal[3 * p], c1 = normal[3 * p + 1], c2 = normal[3 * p + 2];
+    float n0 = N[0] * c0 + N[1] * c1 + N[2] * c2;
+    float n1 = N[3] * c0 + N[4] * c1 + N[5] * c2;
+    float n2 = N[6] * c0 + N[7] * c1 + N[8] * c2;
+    float l = sqrtf(n0 * n0 + n1 * n1 + n2 * n2);
+    if (!(d > 0.f) || !(l > 0.f)) {
+        d = 0.5f * (V.depth_min + V.depth_max);
+        n0 = 0.f; n1 = 0.f; n2 = 1.f; l = 1.f;
+    }
+    n0 /= l; n1 /= l; n2 /= l;
+    if (n2 < 0.f) { n0 = -n0; n1 = -n1; n2 = -n2; }
+    V.hyp[p] = make_float4(d, n0, n1, n2);
+}
+
+// Device-side schedule step: every active view advances one half-step.
+__global__ void k_advance(ViewDesc *views, int nviews)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < nviews && views[v].active && views[v].step < views[v].num_steps) views[v].step += 1;
+}
+
+// Quad-packed texture: texel (x,y) holds I(x,y), I(x1,y), I(x,y1), I(x1,y1)
+// with x1 = min(x+1, W-1), y1 = min(y+1, H-1) -- one dword per bilinear tap.
+__global__ void k_pack_quad(const uint8_t *__restrict__ gray, int W, int H,
+                            uint32_t *__restrict__ quad)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= W || y >= H) return;
+    const int x1 = min(x + 1, W - 1), y1 = min(y + 1, H - 1);
+    const uint32_t a = gray[y * W + x], b = gray[y * W + x1];
+    const uint32_t c = gray[y1 * W + x], d = gray[y1 * W + x1];
+    quad[y * W + x] = a | (b << 8) | (c << 16) | (d << 24);
+}
+
+__global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_halfstep(int maxs, const ViewDesc *views, const Tile *tiles, int ntiles,
+                           const PMConst &c, Counters *counters, hipStream_t st)
+{
+    if (ntiles <= 0) return hipSuccess;
+    if (maxs <= 8)
+        k_halfstep<8><<<ntiles, kThreads, 0, st>>>(views, tiles, ntiles, c, counters);
+    else if (maxs <= 16)
+        k_halfstep<16><<<ntiles, kThreads, 0, st>>>(views, tiles, ntiles, c, counters);
+    else
+        k_halfstep<32><<<ntiles, kThreads, 0, st>>>(views, tiles, ntiles, c, counters);
+    return hipGetLastError();
+}
+
+hipError_t launch_init(const ViewDesc *views, const Tile *tiles, int ntiles, const PMConst &c,
+                       int random_hyp, hipStream_t st)
+{
+    if (ntiles <= 0) return hipSuccess;
+    k_init<<<ntiles, kThreads, 0, st>>>(views, tiles, c, random_hyp);
+    return hipGetLastError();
+}
+
+hipError_t launch_eval_cost(int maxs, const ViewDesc *views, const Tile *tiles, int ntiles,
+                            const PMConst &c, float *out, hipStream_t st)
+{
+    if (ntiles <= 0) return hipSuccess;
+    if (maxs <= 8) k_eval_cost<8><<<ntiles, kThreads, 0, st>>>(views, tiles, c, out);
+    else if (maxs <= 16) k_eval_cost<16><<<ntiles, kThreads, 0, st>>>(views, tiles, c, out);
+    else k_eval_cost<32><<<ntiles, kThreads, 0, st>>>(views, tiles, c, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_filter(int maxs, const ViewDesc *views, const Tile *tiles, int ntiles,
+                         const PMConst &c, uint32_t *mask, hipStream_t st)
+{
+    if (ntiles <= 0) return hipSuccess;
+    if (maxs <= 8) k_filter<8><<<ntiles, kThreads, 0, st>>>(views, tiles, c, mask);
+    else if (maxs <= 16) k_filter<16><<<ntiles, kThreads, 0, st>>>(views, tiles, c, mask);
+    else k_filter<32><<<ntiles, kThreads, 0, st>>>(views, tiles, c, mask);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(const ViewDesc *views, int view, int npix, float *depth, float *normal,
+                           float *cost, float *depth_keep, hipStream_t st)
+{
+    k_finalize<<<(npix + 255) / 256, 256, 0, st>>>(views, view, depth, normal, cost, depth_keep);
+    return hipGetLastError();
+}
+
+hipError_t launch_load_hyp(const ViewDesc *views, int view, int npix, const float *depth,
+                           const float *normal, hipStream_t st)
+{
+    k_load_hyp<<<(npix + 255) / 256, 256, 0, st>>>(views, view, depth, normal);
+    return hipGetLastError();
+}
+
+hipError_t launch_advance(ViewDesc *views, int nviews, hipStream_t st)
+{
+    k_advance<<<(nviews + 63) / 64, 64, 0, st>>>(views, nviews);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_quad(const uint8_t *gray, int W, int H, uint32_t *quad, hipStream_t st)
+{
+    dim3 grid((W + 255) / 256, H);
+    k_pack_quad<<<grid, 256, 0, st>>>(gray, W, H, quad);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(unsigned long long *p, int n, unsigned long long v, hipStream_t st)
+{
+    k_fill_u64<<<(n + 255) / 256, 256, 0, st>>>(p, n, v);
+    return hipGetLastError();
+}
+
+}  // namespace vsm

@@ -1,0 +1,244 @@
+"""Python face of libvissat_mvs.so: a device context holding images, depth
+maps and hypotheses in HBM, with batched solves, serving and DSM fusion.
+
+Every method calls the HIP library through the C ABI; there is no CPU path.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import VsmError, fptr, u32ptr, make_req
+
+
+class MvsContext:
+    """One device context (``vsm_create``).
+
+    Parameters mirror ``PatchMatchStereo.*`` flags (``vsm_params``); the
+    defaults are the reference's photometric call
+    (``colmap_mvs_commands.py:40-48``).
+    """
+
+    def __init__(self, device=0, hbm_budget_bytes=200 * 10**9, **params):
+        self.lib = _lib.load()
+        h = ctypes.c_void_p()
+        rc = self.lib.vsm_create(int(device), int(hbm_budget_bytes), ctypes.byref(h))
+        if rc != 0:
+            raise VsmError(rc, "vsm_create failed (is a HIP device present?)")
+        self.h = h
+        self.sizes = {}
+        self.set_params(**params)
+
+    # ------------------------------------------------------------------
+    def _check(self, rc):
+        if rc != 0:
+            msg = self.lib.vsm_last_error(self.h)
+            raise VsmError(rc, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.vsm_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self):
+        p = _lib.Params()
+        self._check(self.lib.vsm_get_params(self.h, ctypes.byref(p)))
+        return p
+
+    def set_params(self, **kw):
+        p = self.params
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise KeyError(f"unknown parameter {k}")
+            setattr(p, k, v)
+        self._check(self.lib.vsm_set_params(self.h, ctypes.byref(p)))
+
+    # ------------------------------------------------------------------
+    def add_image(self, image_id, gray, P4):
+        P = np.ascontiguousarray(P4, dtype=np.float64).reshape(16)
+        if gray is None:
+            raise ValueError("use register_geometry for images without pixels")
+        g = np.ascontiguousarray(gray, dtype=np.uint8)
+        if g.ndim != 2:
+            raise ValueError("gray image must be 2-D uint8")
+        h, w = g.shape
+        self._check(self.lib.vsm_add_image(self.h, int(image_id),
+                                           g.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                           w, h, w, P.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        self.sizes[int(image_id)] = (h, w)
+
+    def register_geometry(self, image_id, width, height, P4):
+        P = np.ascontiguousarray(P4, dtype=np.float64).reshape(16)
+        self._check(self.lib.vsm_add_image(self.h, int(image_id), None, int(width), int(height),
+                                           int(width), P.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        self.sizes[int(image_id)] = (int(height), int(width))
+
+    def remove_image(self, image_id):
+        self._check(self.lib.vsm_remove_image(self.h, int(image_id)))
+        self.sizes.pop(int(image_id), None)
+
+    def set_depth_map(self, image_id, depth, kind=0):
+        d = np.ascontiguousarray(depth, dtype=np.float32)
+        h, w = d.shape
+        self._check(self.lib.vsm_set_depth_map(self.h, int(image_id), int(kind), fptr(d), w, h))
+
+    # ------------------------------------------------------------------
+    def solve(self, views, pass_=0, want=("depth", "normal", "cost", "consistency"),
+              use_graph=False):
+        """views: list of dicts with ref, srcs, depth_range and optionally
+        init_depth / init_normal.  Returns one dict of outputs per view."""
+        n = len(views)
+        reqs = (_lib.ViewReq * n)()
+        outs = (_lib.ViewOut * n)()
+        keep, results = [], []
+        for i, v in enumerate(views):
+            r, k = make_req(v["ref"], v["srcs"], v["depth_range"], v.get("init_depth"),
+                            v.get("init_normal"))
+            reqs[i] = r
+            keep.append(k)
+            h, w = self.sizes[int(v["ref"])]
+            res = {}
+            if "depth" in want:
+                res["depth"] = np.zeros((h, w), np.float32)
+                outs[i].depth = fptr(res["depth"])
+            if "normal" in want:
+                res["normal"] = np.zeros((h, w, 3), np.float32)
+                outs[i].normal = fptr(res["normal"])
+            if "cost" in want:
+                res["cost"] = np.zeros((h, w), np.float32)
+                outs[i].cost = fptr(res["cost"])
+            if "consistency" in want:
+                res["consistency"] = np.zeros((h, w), np.uint32)
+                outs[i].consistency = u32ptr(res["consistency"])
+            results.append(res)
+        self._check(self.lib.vsm_solve(self.h, reqs, n, int(pass_), outs, 1 if use_graph else 0))
+        return results
+
+    def eval_cost(self, ref, srcs, depth_range, depth, normal_uvd, pass_=0):
+        r, keep = make_req(ref, srcs, depth_range)
+        h, w = self.sizes[int(ref)]
+        d = np.ascontiguousarray(depth, dtype=np.float32).reshape(h, w)
+        nn = np.ascontiguousarray(normal_uvd, dtype=np.float32).reshape(h, w, 3)
+        out = np.zeros((len(srcs), h, w), np.float32)
+        self._check(self.lib.vsm_eval_cost(self.h, ctypes.byref(r), int(pass_), fptr(d), fptr(nn),
+                                           fptr(out)))
+        return out
+
+    def debug_init(self, ref, srcs, depth_range, pass_=0):
+        r, keep = make_req(ref, srcs, depth_range)
+        h, w = self.sizes[int(ref)]
+        st = dict(depth=np.zeros(h * w, np.float32), normal=np.zeros(3 * h * w, np.float32),
+                  cost=np.zeros(h * w, np.float32), prob=np.zeros(len(srcs) * h * w, np.float32))
+        self._check(self.lib.vsm_debug_init(self.h, ctypes.byref(r), int(pass_), fptr(st["depth"]),
+                                            fptr(st["normal"]), fptr(st["cost"]), fptr(st["prob"])))
+        return st
+
+    def debug_halfstep(self, ref, srcs, depth_range, state, pass_, it, color):
+        """One half-step on a copy of state (dict of float32 arrays)."""
+        r, keep = make_req(ref, srcs, depth_range)
+        st = {k: np.ascontiguousarray(v, dtype=np.float32).copy() for k, v in state.items()
+              if k in ("depth", "normal", "cost", "prob")}
+        self._check(self.lib.vsm_debug_halfstep(self.h, ctypes.byref(r), int(pass_), int(it),
+                                                int(color), fptr(st["depth"]), fptr(st["normal"]),
+                                                fptr(st["cost"]), fptr(st["prob"])))
+        return st
+
+    def debug_filter(self, ref, srcs, depth_range, state, pass_):
+        r, keep = make_req(ref, srcs, depth_range)
+        st = {k: np.ascontiguousarray(state[k], dtype=np.float32).copy() for k in ("depth", "normal")}
+        mask = np.zeros(st["depth"].size, np.uint32)
+        self._check(self.lib.vsm_debug_filter(self.h, ctypes.byref(r), int(pass_), fptr(st["depth"]),
+                                              fptr(st["normal"]), u32ptr(mask)))
+        st["mask"] = mask
+        return st
+
+    # ------------------------------------------------------------------
+    def fuse_dsm(self, grid, image_ids, depth_kind=1, min_measurements=3, inv_proj_mats=None,
+                 per_view=False):
+        g = _lib.DsmGrid(*grid) if not isinstance(grid, _lib.DsmGrid) else grid
+        ids = np.ascontiguousarray(image_ids, dtype=np.int32)
+        out = np.zeros((g.n_size, g.e_size), np.float32)
+        pv = np.zeros((len(ids), g.n_size, g.e_size), np.float32) if per_view else None
+        mats = None
+        if inv_proj_mats is not None:
+            mats = np.ascontiguousarray(inv_proj_mats, dtype=np.float64).reshape(len(ids), 16)
+        self._check(self.lib.vsm_fuse_dsm(
+            self.h, ctypes.byref(g), ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(ids),
+            int(depth_kind), int(min_measurements),
+            mats.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if mats is not None else None,
+            fptr(out), fptr(pv)))
+        return (out, pv) if per_view else out
+
+    def proj_to_grid(self, points, grid):
+        g = _lib.DsmGrid(*grid) if not isinstance(grid, _lib.DsmGrid) else grid
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros((g.n_size, g.e_size), np.float64)
+        self._check(self.lib.vsm_proj_to_grid(
+            self.h, pts.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if len(pts) else None,
+            len(pts), ctypes.byref(g), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def median_blur3(self, img):
+        a = np.ascontiguousarray(img, dtype=np.float32)
+        out = np.zeros_like(a)
+        n_size, e_size = a.shape
+        self._check(self.lib.vsm_median_blur3(self.h, fptr(a), e_size, n_size, fptr(out)))
+        return out
+
+    # ------------------------------------------------------------------
+    def submit(self, ref, srcs, depth_range):
+        r, keep = make_req(ref, srcs, depth_range)
+        t = ctypes.c_uint64()
+        self._check(self.lib.vsm_submit(self.h, ctypes.byref(r), ctypes.byref(t)))
+        return t.value
+
+    def serve_step(self):
+        na, nq = ctypes.c_int32(), ctypes.c_int32()
+        self._check(self.lib.vsm_serve_step(self.h, ctypes.byref(na), ctypes.byref(nq)))
+        return na.value, nq.value
+
+    def poll(self, ticket, ref, want=("depth",)):
+        h, w = self.sizes[int(ref)]
+        out = _lib.ViewOut()
+        res = {}
+        if "depth" in want:
+            res["depth"] = np.zeros((h, w), np.float32)
+            out.depth = fptr(res["depth"])
+        if "normal" in want:
+            res["normal"] = np.zeros((h, w, 3), np.float32)
+            out.normal = fptr(res["normal"])
+        if "cost" in want:
+            res["cost"] = np.zeros((h, w), np.float32)
+            out.cost = fptr(res["cost"])
+        done = ctypes.c_int32()
+        self._check(self.lib.vsm_poll(self.h, ctypes.c_uint64(ticket), ctypes.byref(out),
+                                      ctypes.byref(done)))
+        return res if done.value else None
+
+    # ------------------------------------------------------------------
+    def stats(self):
+        s = _lib.Stats()
+        self._check(self.lib.vsm_get_stats(self.h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def reset_stats(self):
+        self._check(self.lib.vsm_reset_stats(self.h))
+
+    def set_timing(self, on=True):
+        self._check(self.lib.vsm_set_timing(self.h, 1 if on else 0))
+
+    def synchronize(self):
+        self._check(self.lib.vsm_synchronize(self.h))

@@ -521,7 +521,7 @@ int run_views(vsm_ctx *c, std::vector<ViewState> &views, const std::vector<const
                 if ((rc = dalloc(c, &tmp, hw * 4))) { cleanup(); return rc; }
                 RUNCHK(hipMemcpyAsync(tmp, r.init_depth, hw * 4, hipMemcpyHostToDevice, c->stream));
                 RUNCHK(hipMemcpyAsync(tmp + hw, r.init_normal, hw * 12, hipMemcpyHostToDevice, c->stream));
-                RUNCHK(launch_load_hyp(dviews, v, (int)hw, tmp, tmp + hw, c->stream));
+                RUNCHK(launch_load_hyp(dviews, v, (int)hw, tmp, tmp + hw, pc.cos_max_normal, c->stream));
                 RUNCHK(hipStreamSynchronize(c->stream));
                 dfree(c, tmp);
             } else {

@@ -23,6 +23,8 @@
 
 namespace vsm {
 
+constexpr int kGroupA = 4;      // candidate hypotheses per shared window pass (4 + 3)
+
 struct SrcLds {
     float A[12];                 // tile-centred ref -> src (anchor-relative), normalised
     float B[12];                 // src (anchor-relative) -> ref (tile-centre-relative)
@@ -100,13 +102,17 @@ __device__ __forceinline__ Proj make_proj(const SrcLds &S, float du, float dv, f
     return P;
 }
 
+typedef const __attribute__((address_space(1))) uint32_t *gquad_t;
+typedef const __attribute__((address_space(1))) float *gfloat_t;
+
 // Bilinear sample (DN) at anchor-relative (px, py), already clamped in-image.
+// The texel index fits 24 bits x 16 bits (images <= 32768 x 32768, <= 2^30 px).
 __device__ __forceinline__ float sample_quad(const SrcLds &S, float px, float py)
 {
     const float fx0 = floorf(px), fy0 = floorf(py);
     const float fx = px - fx0, fy = py - fy0;
-    const int ix = S.ax + (int)fx0, iy = S.ay + (int)fy0;
-    const uint32_t v = S.quad[iy * S.W + ix];
+    const unsigned ix = (unsigned)(S.ax + (int)fx0), iy = (unsigned)(S.ay + (int)fy0);
+    const uint32_t v = ((gquad_t)S.quad)[__umul24(iy, (unsigned)S.W) + ix];
     const float a = (float)(v & 0xFFu), b = (float)((v >> 8) & 0xFFu);
     const float c = (float)((v >> 16) & 0xFFu), d = (float)(v >> 24);
     const float top = fmaf(fx, b - a, a);
@@ -114,9 +120,11 @@ __device__ __forceinline__ float sample_quad(const SrcLds &S, float px, float py
     return fmaf(fy, bot - top, top);
 }
 
+// Bilateral weight of a tap: 2^-(r^2 ccc + (dx^2+dy^2) css), the hardware
+// exp2 (weights below 2^-126 flush to zero, which no sum can notice).
 __device__ __forceinline__ float tap_weight(const PMConst &c, float r, int dx, int dy)
 {
-    return exp2f(-fmaf(r * r, c.ccc, (float)(dx * dx + dy * dy) * c.css));
+    return __builtin_amdgcn_exp2f(-fmaf(r * r, c.ccc, (float)(dx * dx + dy * dy) * c.css));
 }
 
 struct RefStats {
@@ -146,49 +154,114 @@ __device__ __forceinline__ RefStats ref_stats(const PMConst &c, const float *s_r
     return st;
 }
 
-// 1 - bilaterally weighted NCC of the plane P against source S.
-__device__ float ncc_cost(const PMConst &c, const SrcLds &S, const Proj &P, const float *s_ref,
-                          int lbase, const RefStats &R)
+// Largest relative change of the homogeneous depth q.z across a window for
+// which the per-tap reciprocal is replaced by one Newton step from the exact
+// centre reciprocal: iz = iz0 (2 - qz iz0), relative error delta^2 <= 1e-6.
+// Satellite cameras (600 km range) give delta ~ 1e-5.
+constexpr float kDeltaMax = 1e-3f;
+
+// Window pass of G plane hypotheses against the same source S: the bilateral
+// weight and the reference value of each tap are computed once and shared;
+// the G gathers per tap are independent.  EXACT selects the per-tap
+// reciprocal (fallback when some lane's window exceeds kDeltaMax).
+template <int G, bool EXACT>
+__device__ __forceinline__ void window_pass(const PMConst &c, const SrcLds &S, const Proj (&P)[G],
+                                            const float (&iz0)[G], const float (&sc)[G],
+                                            const float *s_ref, int lbase, const RefStats &R,
+                                            float (&ss)[G], float (&sss)[G], float (&srs)[G])
 {
-    if (!(P.q0z > 0.f)) return kMaxCost;
-    const float cx = P.q0x / P.q0z, cy = P.q0y / P.q0z;
-    if (!(cx >= S.lox && cx <= S.hix && cy >= S.loy && cy <= S.hiy)) return kMaxCost;
-    const float sc = sample_quad(S, cx, cy);
-    float ss = 0.f, sss = 0.f, srs = 0.f;
     const int rad = c.radius, st = c.step, nt = c.ntaps;
+    const float frad = (float)(-rad);
+    float giu[G], giv[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        giu[k] = -iz0[k] * iz0[k] * P[k].huz;
+        giv[k] = -iz0[k] * iz0[k] * P[k].hvz;
+        ss[k] = 0.f; sss[k] = 0.f; srs[k] = 0.f;
+    }
     for (int j = 0; j < nt; ++j) {
         const int dy = -rad + j * st;
-        const float fdy = (float)dy, frad = (float)(-rad);
-        const float rx = fmaf(fdy, P.hvx, fmaf(frad, P.hux, P.q0x));
-        const float ry = fmaf(fdy, P.hvy, fmaf(frad, P.huy, P.q0y));
-        const float rz = fmaf(fdy, P.hvz, fmaf(frad, P.huz, P.q0z));
+        const float fdy = (float)dy;
+        float rx[G], ry[G], rz[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            rx[k] = fmaf(fdy, P[k].hvx, fmaf(frad, P[k].hux, P[k].q0x));
+            ry[k] = fmaf(fdy, P[k].hvy, fmaf(frad, P[k].huy, P[k].q0y));
+            rz[k] = EXACT ? fmaf(fdy, P[k].hvz, fmaf(frad, P[k].huz, P[k].q0z))
+                          : fmaf(fdy, giv[k], fmaf(frad, giu[k], iz0[k]));
+        }
         const float *row = s_ref + lbase + dy * kLdsW;
-#pragma unroll 2
+#pragma unroll 1
         for (int i = 0; i < nt; ++i) {
             const int dx = -rad + i * st;
             const float fi = (float)(i * st);
-            const float qx = fmaf(fi, P.hux, rx);
-            const float qy = fmaf(fi, P.huy, ry);
-            const float qz = fmaf(fi, P.huz, rz);
-            const float iz = __builtin_amdgcn_rcpf(qz);
-            float px = qx * iz, py = qy * iz;
-            px = fminf(fmaxf(px, S.lox), S.hix);
-            py = fminf(fmaxf(py, S.loy), S.hiy);
-            const float sv = sample_quad(S, px, py) - sc;
             const float r = row[dx] - R.rc;
             const float w = tap_weight(c, r, dx, dy);
-            const float ws = w * sv;
-            ss += ws;
-            sss = fmaf(ws, sv, sss);
-            srs = fmaf(w * r, sv, srs);
+            const float wr = w * r;
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                const float qx = fmaf(fi, P[k].hux, rx[k]);
+                const float qy = fmaf(fi, P[k].huy, ry[k]);
+                const float iz = EXACT ? __builtin_amdgcn_rcpf(fmaf(fi, P[k].huz, rz[k]))
+                                       : fmaf(fi, giu[k], rz[k]);
+                // hypotheses are finite (NaN-free), so med3 is a plain clamp
+                const float px = __builtin_amdgcn_fmed3f(qx * iz, S.lox, S.hix);
+                const float py = __builtin_amdgcn_fmed3f(qy * iz, S.loy, S.hiy);
+                const float sv = sample_quad(S, px, py) - sc[k];
+                const float ws = w * sv;
+                ss[k] += ws;
+                sss[k] = fmaf(ws, sv, sss[k]);
+                srs[k] = fmaf(wr, sv, srs[k]);
+            }
         }
     }
-    const float ms = ss / R.sw;
-    const float var_s = sss / R.sw - ms * ms;
-    if (R.var_r < c.min_var || var_s < c.min_var) return kMaxCost;
-    const float cov = srs / R.sw - R.mr * ms;
-    const float ncc = cov / sqrtf(R.var_r * var_s);
-    return fminf(fmaxf(1.f - ncc, 0.f), kMaxCost);
+}
+
+// NCC cost of G plane hypotheses against source S (same arithmetic per
+// hypothesis as ncc_cost up to the reciprocal of the window pass).
+template <int G>
+__device__ __forceinline__ void ncc_cost_group(const PMConst &c, const SrcLds &S, const Proj (&P)[G],
+                                               const float *s_ref, int lbase, const RefStats &R,
+                                               float (&cost)[G])
+{
+    bool valid[G];
+    float sc[G], iz0[G], ss[G], sss[G], srs[G];
+    bool far = false;
+    const float fr = (float)c.radius;
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        valid[k] = P[k].q0z > 0.f;
+        iz0[k] = valid[k] ? 1.f / P[k].q0z : 1.f;
+        const float cx = P[k].q0x * iz0[k], cy = P[k].q0y * iz0[k];
+        valid[k] = valid[k] && cx >= S.lox && cx <= S.hix && cy >= S.loy && cy <= S.hiy;
+        sc[k] = sample_quad(S, __builtin_amdgcn_fmed3f(cx, S.lox, S.hix),
+                            __builtin_amdgcn_fmed3f(cy, S.loy, S.hiy));
+        const float delta = fr * (fabsf(P[k].huz) + fabsf(P[k].hvz)) * fabsf(iz0[k]);
+        far = far || !(delta <= kDeltaMax);
+    }
+    if (__ballot(far) != 0ull)
+        window_pass<G, true>(c, S, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
+    else
+        window_pass<G, false>(c, S, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        const float ms = ss[k] / R.sw;
+        const float var_s = sss[k] / R.sw - ms * ms;
+        const float cov = srs[k] / R.sw - R.mr * ms;
+        const bool flat = R.var_r < c.min_var || var_s < c.min_var;
+        const float ncc = cov / sqrtf(R.var_r * var_s);
+        cost[k] = (!valid[k] || flat) ? kMaxCost : fminf(fmaxf(1.f - ncc, 0.f), kMaxCost);
+    }
+}
+
+// 1 - bilaterally weighted NCC of the plane P against source S.
+__device__ __forceinline__ float ncc_cost(const PMConst &c, const SrcLds &S, const Proj &P,
+                                          const float *s_ref, int lbase, const RefStats &R)
+{
+    const Proj PP[1] = {P};
+    float cost[1];
+    ncc_cost_group<1>(c, S, PP, s_ref, lbase, R, cost);
+    return cost[0];
 }
 
 // Forward-backward reprojection error (px) through the source depth map.
@@ -203,8 +276,9 @@ __device__ float geo_error(const PMConst &c, const SrcLds &S, const Proj &P, flo
     const int x0 = S.ax + (int)fx0, y0 = S.ay + (int)fy0;
     const int x1 = min(x0 + 1, S.W - 1), y1 = min(y0 + 1, S.H - 1);
     const float fx = px - fx0, fy = py - fy0;
-    const float da = S.depth[y0 * S.W + x0], db = S.depth[y0 * S.W + x1];
-    const float dc = S.depth[y1 * S.W + x0], de = S.depth[y1 * S.W + x1];
+    const gfloat_t D = (gfloat_t)S.depth;
+    const float da = D[y0 * S.W + x0], db = D[y0 * S.W + x1];
+    const float dc = D[y1 * S.W + x0], de = D[y1 * S.W + x1];
     if (!(da > 0.f && db > 0.f && dc > 0.f && de > 0.f)) return gmax;
     const float top = fmaf(fx, db - da, da), bot = fmaf(fx, de - dc, dc);
     const float ds = fmaf(fy, bot - top, top);
@@ -474,26 +548,49 @@ __global__ __launch_bounds__(kThreads) void k_halfstep(const ViewDesc *__restric
         }
 
         // ---- evaluate the other seven hypotheses -------------------------
+        // One pass over the window per sampled source evaluates all seven.
+        float ak[kNHyp - 1], bk[kNHyp - 1], acc[kNHyp - 1];
+#pragma unroll
+        for (int k = 0; k < kNHyp - 1; ++k) {
+            ak[k] = -hn0[k + 1] / hn2[k + 1];
+            bk[k] = -hn1[k + 1] / hn2[k + 1];
+            acc[k] = 0.f;
+        }
+        for (int j = 0; j < nlist; ++j) {
+            const uint32_t e = s_list[j * kThreads + tid];
+            const SrcLds &Ss = s_src[e & 0xFFu];
+            const float cnt = (float)(e >> 8);
+            {   // hypotheses 1..kGroupA
+                Proj P[kGroupA];
+                float ph[kGroupA];
+#pragma unroll
+                for (int k = 0; k < kGroupA; ++k) P[k] = make_proj(Ss, du, dv, hd[k + 1], ak[k], bk[k]);
+                ncc_cost_group<kGroupA>(c, Ss, P, s_ref, lbase, R, ph);
+#pragma unroll
+                for (int k = 0; k < kGroupA; ++k)
+                    acc[k] += cnt * total_cost(c, Ss, P[k], ph[k], pass, du, dv);
+            }
+            {   // hypotheses kGroupA+1..7
+                constexpr int GB = kNHyp - 1 - kGroupA;
+                Proj P[GB];
+                float ph[GB];
+#pragma unroll
+                for (int k = 0; k < GB; ++k)
+                    P[k] = make_proj(Ss, du, dv, hd[kGroupA + 1 + k], ak[kGroupA + k], bk[kGroupA + k]);
+                ncc_cost_group<GB>(c, Ss, P, s_ref, lbase, R, ph);
+#pragma unroll
+                for (int k = 0; k < GB; ++k)
+                    acc[kGroupA + k] += cnt * total_cost(c, Ss, P[k], ph[k], pass, du, dv);
+            }
+        }
+        my_ncc += (kNHyp - 1) * nlist;
+        if (pass == 1) my_geo += (kNHyp - 1) * nlist;
         float best_c = acc0 / wsum;
         int best = 0;
-        for (int k = 1; k < kNHyp; ++k) {
-            const float dk = pick8(k, hd[0], hd[1], hd[2], hd[3], hd[4], hd[5], hd[6], hd[7]);
-            const float n0 = pick8(k, hn0[0], hn0[1], hn0[2], hn0[3], hn0[4], hn0[5], hn0[6], hn0[7]);
-            const float n1 = pick8(k, hn1[0], hn1[1], hn1[2], hn1[3], hn1[4], hn1[5], hn1[6], hn1[7]);
-            const float n2 = pick8(k, hn2[0], hn2[1], hn2[2], hn2[3], hn2[4], hn2[5], hn2[6], hn2[7]);
-            const float ak = -n0 / n2, bk = -n1 / n2;
-            float acc = 0.f;
-            for (int j = 0; j < nlist; ++j) {
-                const uint32_t e = s_list[j * kThreads + tid];
-                const SrcLds &Ss = s_src[e & 0xFFu];
-                const Proj P = make_proj(Ss, du, dv, dk, ak, bk);
-                const float ph = ncc_cost(c, Ss, P, s_ref, lbase, R);
-                acc += (float)(e >> 8) * total_cost(c, Ss, P, ph, pass, du, dv);
-            }
-            my_ncc += nlist;
-            if (pass == 1) my_geo += nlist;
-            const float ck = acc / wsum;
-            if (ck < best_c) { best_c = ck; best = k; }
+#pragma unroll
+        for (int k = 0; k < kNHyp - 1; ++k) {
+            const float ck = acc[k] / wsum;
+            if (ck < best_c) { best_c = ck; best = k + 1; }
         }
         const float bd = pick8(best, hd[0], hd[1], hd[2], hd[3], hd[4], hd[5], hd[6], hd[7]);
         const float b0n = pick8(best, hn0[0], hn0[1], hn0[2], hn0[3], hn0[4], hn0[5], hn0[6], hn0[7]);
@@ -639,7 +736,7 @@ __global__ void k_finalize(const ViewDesc *__restrict__ views, int view, float *
 
 // Hypotheses from depth + camera-frame normals (resume from workspace files).
 __global__ void k_load_hyp(const ViewDesc *__restrict__ views, int view, const float *depth,
-                           const float *normal)
+                           const float *normal, float c_cos_max)
 {
     const ViewDesc &V = views[view];
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -657,6 +754,13 @@ __global__ void k_load_hyp(const ViewDesc *__restrict__ views, int view, const f
     }
     n0 /= l; n1 /= l; n2 /= l;
     if (n2 < 0.f) { n0 = -n0; n1 = -n1; n2 = -n2; }
+    // keep loaded normals inside the cone the solver samples from, so plane
+    // slopes (and every projected coordinate) stay finite
+    if (!(n2 >= c_cos_max)) {
+        const float t = sqrtf(fmaxf(0.f, 1.f - c_cos_max * c_cos_max));
+        const float h = sqrtf(n0 * n0 + n1 * n1);
+        n0 = h > 0.f ? n0 / h * t : t; n1 = h > 0.f ? n1 / h * t : 0.f; n2 = c_cos_max;
+    }
     V.hyp[p] = make_float4(d, n0, n1, n2);
 }
 
@@ -739,9 +843,9 @@ hipError_t launch_finalize(const ViewDesc *views, int view, int npix, float *dep
 }
 
 hipError_t launch_load_hyp(const ViewDesc *views, int view, int npix, const float *depth,
-                           const float *normal, hipStream_t st)
+                           const float *normal, float cos_max_normal, hipStream_t st)
 {
-    k_load_hyp<<<(npix + 255) / 256, 256, 0, st>>>(views, view, depth, normal);
+    k_load_hyp<<<(npix + 255) / 256, 256, 0, st>>>(views, view, depth, normal, cos_max_normal);
     return hipGetLastError();
 }
 

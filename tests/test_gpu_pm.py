@@ -245,3 +245,60 @@ def test_geometric_pass_and_filter(tiny):
         assert np.median(eg) <= np.median(ep) + 0.05
         pc = np.array([bin(int(m)).count("1") for m in ge["consistency"].ravel()]).reshape(valid.shape)
         assert np.all(pc[~valid] < 2) and np.all(pc[valid] >= 2)
+
+
+def _src_depths(site, srcs, seed=5):
+    rng = np.random.default_rng(seed)
+    out = {}
+    for s in srcs:
+        d = sites.gt_depth(site, s).astype(np.float32)
+        d = d + rng.normal(0, 0.2, d.shape).astype(np.float32)
+        d[rng.random(d.shape) < 0.05] = 0.0
+        out[s] = d
+    return out
+
+
+def test_lockstep_geometric_multisource(tiny):
+    names = tiny["names"]
+    srcs = [n for n in names if n != names[0]]
+    p = _params(window_radius=4)
+    rep, st = _lockstep(tiny, p, 0, None, 1, 4, src_depths_gpu=_src_depths(tiny, srcs))
+    for r in rep:
+        assert r["decided_mismatch"] <= 1e-3, r
+        assert r["frac_cost_err"] <= 1e-3, r
+
+
+def test_filter_matches_oracle(tiny):
+    """Consistency masks agree except where a source sits on a threshold."""
+    import ctypes
+    import pm_oracle as po
+    names = tiny["names"]
+    ref, srcs = names[0], names[1:]
+    p = _params(window_radius=4)
+    sd = _src_depths(tiny, srcs)
+    ctx, _, _ = _setup(tiny, p)
+    for s in srcs:
+        ctx.set_depth_map(_id(tiny, s), sd[s], kind=0)
+    ov = _oracle(tiny, ref, srcs, p, src_depths=[sd[s] for s in srcs])
+    st = ov.solve(1, 2, state=None)                      # oracle hypotheses to filter
+    g = ctx.debug_filter(_id(tiny, ref), [_id(tiny, s) for s in srcs], tiny["dr"][ref],
+                         dict(depth=st["depth"].astype(np.float32), normal=st["normal"].astype(np.float32)), 1)
+    so = {k: st[k].copy() for k in ("depth", "normal")}
+    mo = ov.filter(so, 1)
+    diff = np.nonzero(g["mask"].astype(np.uint64) != mo)[0]
+    v, pr = ov._view(), ov._p()
+    W = ov.W
+    for i in diff:
+        y, x = divmod(int(i), W)
+        marg = []
+        for s in range(len(srcs)):
+            c = po.lib().orc_photo_cost(ctypes.byref(v), ctypes.byref(pr), x, y, st["depth"][i],
+                                        st["normal"][3 * i:3 * i + 3].ctypes.data_as(ctypes.POINTER(ctypes.c_double)), s)
+            e = po.lib().orc_geo_error(ctypes.byref(v), ctypes.byref(pr), x, y, st["depth"][i], s)
+            marg.append(min(abs(1 - c - pr.filter_min_ncc), abs(e - pr.filter_geom_max_cost)))
+        assert min(marg) < 1e-3, (x, y, marg)
+    print("filter mask mismatches", len(diff), "of", len(mo), "kept", np.mean(st["depth"] > 0))
+    assert len(diff) <= 1e-3 * len(mo) + 2
+    keep_g = g["depth"] > 0
+    keep_o = so["depth"] > 0
+    assert np.mean(keep_g != keep_o) <= 1e-3

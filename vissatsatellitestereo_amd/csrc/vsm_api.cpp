@@ -75,6 +75,8 @@ struct vsm_ctx {
     std::map<void *, size_t> allocs;
     std::string err;
     Counters *d_counters = nullptr;
+    unsigned int *d_work = nullptr;        // per-XCD tile cursors of the persistent kernel
+    int grid[3] = {0, 0, 0};               // resident workgroups per MAXS variant
     bool timing = false;
     double halfstep_ms = 0.0;
     int64_t halfstep_launches = 0;
@@ -444,11 +446,19 @@ void release_view(vsm_ctx *c, ViewState &vs)
     vs.hyp = nullptr; vs.cost = nullptr; vs.prob = nullptr;
 }
 
+int grid_for(vsm_ctx *c, int maxs)
+{
+    const int k = maxs <= 8 ? 0 : (maxs <= 16 ? 1 : 2);
+    if (!c->grid[k]) c->grid[k] = halfstep_grid(maxs);
+    return c->grid[k];
+}
+
 int record_halfstep(vsm_ctx *c, int maxs, const ViewDesc *dviews, const Tile *dtiles, int ntiles,
                     const PMConst &pc, bool timed)
 {
     if (timed) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    HIPCHK(c, launch_halfstep(maxs, dviews, dtiles, ntiles, pc, c->d_counters, c->stream));
+    HIPCHK(c, launch_halfstep(maxs, dviews, dtiles, ntiles, pc, c->d_counters, c->d_work,
+                              grid_for(c, maxs), c->stream));
     if (timed) {
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
         HIPCHK(c, hipEventSynchronize(c->ev1));
@@ -540,7 +550,8 @@ int run_views(vsm_ctx *c, std::vector<ViewState> &views, const std::vector<const
         RUNCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         hipError_t ce = hipSuccess;
         for (int s = 0; s < nsteps && ce == hipSuccess; ++s) {
-            ce = launch_halfstep(maxs, dviews, dtiles, ntiles, pc, c->d_counters, c->stream);
+            ce = launch_halfstep(maxs, dviews, dtiles, ntiles, pc, c->d_counters, c->d_work,
+                                 grid_for(c, maxs), c->stream);
             if (ce == hipSuccess) ce = launch_advance(dviews, n, c->stream);
         }
         hipError_t ee = hipStreamEndCapture(c->stream, &graph);
@@ -683,7 +694,7 @@ int vsm_create(int device, uint64_t hbm_budget_bytes, vsm_ctx **out)
         delete c;
         return VSM_EDEVICE;
     }
-    if (dalloc(c, &c->d_counters, 1) != VSM_OK) {
+    if (dalloc(c, &c->d_counters, 1) != VSM_OK || dalloc(c, &c->d_work, 8) != VSM_OK) {
         delete c;
         return VSM_ENOMEM;
     }
@@ -888,7 +899,9 @@ static int debug_state(vsm_ctx *c, const vsm_view_req *req, int pass, int mode, 
         HIPCHK(c, hipMemcpyAsync(dt, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream));
         const int nt = (int)tiles.size();
         if (mode == 0) HIPCHK(c, launch_init(dv, dt, nt, pc, 1, c->stream));
-        else if (mode == 1) HIPCHK(c, launch_halfstep(V.n_src, dv, dt, nt, pc, c->d_counters, c->stream));
+        else if (mode == 1)
+            HIPCHK(c, launch_halfstep(V.n_src, dv, dt, nt, pc, c->d_counters, c->d_work,
+                                      grid_for(c, V.n_src), c->stream));
         else HIPCHK(c, launch_filter(V.n_src, dv, dt, nt, pc, dm, c->stream));
         HIPCHK(c, hipMemcpyAsync(h.data(), vs.hyp, hw * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
         if (cost && mode != 2) HIPCHK(c, hipMemcpyAsync(cost, vs.cost, hw * 4, hipMemcpyDeviceToHost, c->stream));

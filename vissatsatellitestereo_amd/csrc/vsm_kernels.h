@@ -5,7 +5,10 @@
 namespace vsm {
 
 hipError_t launch_halfstep(int maxs, const ViewDesc *views, const Tile *tiles, int ntiles,
-                           const PMConst &c, Counters *counters, hipStream_t st);
+                           const PMConst &c, Counters *counters, unsigned int *work, int grid,
+                           hipStream_t st);
+// Resident workgroups of the persistent half-step kernel on this device.
+int halfstep_grid(int maxs);
 hipError_t launch_init(const ViewDesc *views, const Tile *tiles, int ntiles, const PMConst &c,
                        int random_hyp, hipStream_t st);
 hipError_t launch_eval_cost(int maxs, const ViewDesc *views, const Tile *tiles, int ntiles,

@@ -379,31 +379,23 @@ __device__ __forceinline__ void tile_prologue(const ViewDesc &V, const Tile &t, 
         setup_source(V, threadIdx.x, t.x0 + kTileW / 2, t.y0 + kTileH / 2, s_src[threadIdx.x]);
 }
 
+// One red-black half-step on one tile (all 256 threads of the workgroup
+// call it; it contains one barrier).
 template <int MAXS>
-__global__ __launch_bounds__(kThreads) void k_halfstep(const ViewDesc *__restrict__ views,
-                                                       const Tile *__restrict__ tiles, int ntiles,
-                                                       PMConst c, Counters *counters)
+__device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, const PMConst &c,
+                                              float *s_ref, SrcLds *s_src, float *s_c0,
+                                              uint16_t *s_list, unsigned int &my_ncc,
+                                              unsigned int &my_geo)
 {
-    __shared__ float s_ref[kLdsH * kLdsW];
-    __shared__ SrcLds s_src[MAXS];
-    __shared__ float s_c0[MAXS * kThreads];
-    __shared__ uint16_t s_list[MAXS * kThreads];
-    __shared__ unsigned int s_evals[2];
-
-    const Tile t = tiles[blockIdx.x];
-    const ViewDesc &V = views[t.view];
-    if (!V.active || V.step >= V.num_steps) return;       // uniform per workgroup
     const int iter = V.step >> 1, color = V.step & 1, pass = V.pass;
     const int W = V.W, H = V.H, S = V.n_src;
     const int tid = threadIdx.x;
-    if (tid < 2) s_evals[tid] = 0u;
     tile_prologue<MAXS>(V, t, s_ref, s_src);
     __syncthreads();
 
     const int ty = tid >> 4;
     const int tx = ((tid & 15) << 1) + ((t.y0 + ty + color) & 1);
     const int x = t.x0 + tx, y = t.y0 + ty;
-    unsigned int my_ncc = 0, my_geo = 0;
     if (x < W && y < H) {
         const size_t HW = (size_t)W * H;
         const int p = y * W + x;
@@ -599,6 +591,51 @@ __global__ __launch_bounds__(kThreads) void k_halfstep(const ViewDesc *__restric
         V.hyp[p] = make_float4(bd, b0n, b1n, b2n);
         V.cost[p] = best_c;
     }
+}
+
+constexpr int kXcd = 8;   // MI355X: 8 XCDs, workgroups dispatched round-robin across them
+
+// Persistent red-black half-step over a tile list.  Workgroup b runs on XCD
+// b % 8 and takes tiles from that XCD's contiguous share of the list (the
+// views' source images stay in that XCD's L2), then steals from the other
+// XCDs' shares.  work[8] are per-XCD cursors zeroed before the launch.  The
+// order tiles are taken in never changes a result: tiles are independent
+// within a half-step.
+template <int MAXS>
+__global__ __launch_bounds__(kThreads) void k_halfstep(const ViewDesc *__restrict__ views,
+                                                       const Tile *__restrict__ tiles, int ntiles,
+                                                       PMConst c, Counters *counters,
+                                                       unsigned int *work)
+{
+    __shared__ float s_ref[kLdsH * kLdsW];
+    __shared__ SrcLds s_src[MAXS];
+    __shared__ float s_c0[MAXS * kThreads];
+    __shared__ uint16_t s_list[MAXS * kThreads];
+    __shared__ unsigned int s_evals[2];
+    __shared__ int s_tile;
+    const int tid = threadIdx.x;
+    if (tid < 2) s_evals[tid] = 0u;
+    unsigned int my_ncc = 0, my_geo = 0;
+    const int chunk = (ntiles + kXcd - 1) / kXcd;
+    const int home = blockIdx.x % kXcd;
+    for (int probe = 0; probe < kXcd; ++probe) {
+        const int xcd = (home + probe) % kXcd;
+        const int lo = xcd * chunk, hi = min(ntiles, lo + chunk);
+        while (true) {
+            if (tid == 0) {
+                const int i = (int)atomicAdd(&work[xcd], 1u);
+                s_tile = lo + i < hi ? lo + i : -1;
+            }
+            __syncthreads();
+            const int ti = s_tile;
+            __syncthreads();
+            if (ti < 0) break;
+            const Tile t = tiles[ti];
+            const ViewDesc &V = views[t.view];
+            if (!V.active || V.step >= V.num_steps) continue;     // uniform per workgroup
+            halfstep_tile<MAXS>(V, t, c, s_ref, s_src, s_c0, s_list, my_ncc, my_geo);
+        }
+    }
     if (counters) {
         atomicAdd(&s_evals[0], my_ncc);
         atomicAdd(&s_evals[1], my_geo);
@@ -606,7 +643,6 @@ __global__ __launch_bounds__(kThreads) void k_halfstep(const ViewDesc *__restric
         if (tid == 0) {
             atomicAdd(&counters->ncc_evals, (unsigned long long)s_evals[0]);
             atomicAdd(&counters->geo_evals, (unsigned long long)s_evals[1]);
-            atomicAdd(&counters->tiles, 1ull);
         }
     }
 }
@@ -795,16 +831,33 @@ __global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)
 // Host launchers
 // ---------------------------------------------------------------------------
 hipError_t launch_halfstep(int maxs, const ViewDesc *views, const Tile *tiles, int ntiles,
-                           const PMConst &c, Counters *counters, hipStream_t st)
+                           const PMConst &c, Counters *counters, unsigned int *work, int grid,
+                           hipStream_t st)
 {
     if (ntiles <= 0) return hipSuccess;
+    hipError_t e = launch_fill_u32(work, kXcd, 0u, st);
+    if (e != hipSuccess) return e;
+    grid = max(kXcd, min(grid, ((ntiles + kXcd - 1) / kXcd) * kXcd));
     if (maxs <= 8)
-        k_halfstep<8><<<ntiles, kThreads, 0, st>>>(views, tiles, ntiles, c, counters);
+        k_halfstep<8><<<grid, kThreads, 0, st>>>(views, tiles, ntiles, c, counters, work);
     else if (maxs <= 16)
-        k_halfstep<16><<<ntiles, kThreads, 0, st>>>(views, tiles, ntiles, c, counters);
+        k_halfstep<16><<<grid, kThreads, 0, st>>>(views, tiles, ntiles, c, counters, work);
     else
-        k_halfstep<32><<<ntiles, kThreads, 0, st>>>(views, tiles, ntiles, c, counters);
+        k_halfstep<32><<<grid, kThreads, 0, st>>>(views, tiles, ntiles, c, counters, work);
     return hipGetLastError();
+}
+
+int halfstep_grid(int maxs)
+{
+    int dev = 0, ncu = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 1024;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) ncu = 256;
+    hipError_t e;
+    if (maxs <= 8) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_halfstep<8>, kThreads, 0);
+    else if (maxs <= 16) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_halfstep<16>, kThreads, 0);
+    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_halfstep<32>, kThreads, 0);
+    if (e != hipSuccess || per_cu < 1) per_cu = 1;
+    return ((ncu * per_cu + kXcd - 1) / kXcd) * kXcd;
 }
 
 hipError_t launch_init(const ViewDesc *views, const Tile *tiles, int ntiles, const PMConst &c,

@@ -1,0 +1,94 @@
+"""Serving loop: per-view depth requests arriving over time, continuously
+batched on one device.
+
+The engine (``MvsContext`` or any object with ``submit(view) -> ticket``,
+``serve_step() -> (n_active, n_queued)`` and ``poll(ticket, ref) -> result |
+None``) admits a request when its HBM footprint fits the context's budget,
+adds it to the running batch at the next half-step boundary and keeps its
+image, hypotheses and view-selection state resident until it finishes.  One
+``serve_step`` is one batched half-step launch over every admitted request,
+each at its own iteration.  Requests that arrive while others run join as
+slots free up; a request's result does not depend on its co-runners (per-view
+counter-based random streams, tiles independent within a half-step).
+
+``run_schedule`` replays an arrival schedule against wall-clock time and
+reports p50/p99 latency (arrival -> result available) and views/s.
+"""
+import math
+import time
+
+import numpy as np
+
+
+def poisson_arrivals(n, rate_per_s, seed=6, t0=0.0):
+    """Arrival times of n requests with exponential inter-arrival gaps."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate_per_s, n)
+    return list(t0 + np.cumsum(gaps))
+
+
+def percentile(values, q):
+    return float(np.percentile(np.asarray(values, dtype=np.float64), q)) if values else math.nan
+
+
+class ServeLoop:
+    def __init__(self, engine, clock=time.perf_counter, want=("depth",)):
+        self.engine = engine
+        self.clock = clock
+        self.want = want
+        self.pending = {}      # ticket -> (view, arrival time)
+        self.done = {}         # ticket -> (result, arrival, completion)
+
+    def submit(self, view, arrival=None):
+        t = self.engine.submit(view["ref"], view["srcs"], view["depth_range"])
+        self.pending[t] = (view, self.clock() if arrival is None else arrival)
+        return t
+
+    def step(self):
+        """One batched half-step; collects finished requests."""
+        n_active, n_queued = self.engine.serve_step()
+        now = self.clock()
+        for t in list(self.pending):
+            view, arr = self.pending[t]
+            res = self.engine.poll(t, view["ref"], self.want)
+            if res is not None:
+                self.done[t] = (res, arr, now)
+                del self.pending[t]
+        return n_active, n_queued
+
+    def drain(self, max_steps=10**7):
+        k = 0
+        while self.pending and k < max_steps:
+            self.step()
+            k += 1
+        if self.pending:
+            raise RuntimeError(f"{len(self.pending)} requests still pending after {max_steps} steps")
+
+    def run_schedule(self, arrivals, views, idle_sleep=0.0005, max_steps=10**7):
+        """arrivals[i] (seconds from start) is when views[i] is submitted."""
+        order = np.argsort(arrivals)
+        start = self.clock()
+        nxt = 0
+        tickets = [None] * len(views)
+        steps = 0
+        while nxt < len(order) or self.pending:
+            now = self.clock() - start
+            while nxt < len(order) and arrivals[order[nxt]] <= now:
+                i = int(order[nxt])
+                tickets[i] = self.submit(views[i], arrival=start + arrivals[i])
+                nxt += 1
+            if self.pending:
+                self.step()
+                steps += 1
+                if steps > max_steps:
+                    raise RuntimeError("serving did not drain")
+            elif nxt < len(order):
+                time.sleep(min(idle_sleep, max(0.0, arrivals[order[nxt]] - now)))
+        wall = self.clock() - start
+        lat = [self.done[t][2] - self.done[t][1] for t in tickets]
+        return dict(tickets=tickets, latency_s=lat, wall_s=wall, steps=steps,
+                    p50_ms=1e3 * percentile(lat, 50), p99_ms=1e3 * percentile(lat, 99),
+                    views_per_s=len(views) / wall if wall > 0 else math.nan)
+
+    def result(self, ticket):
+        return self.done[ticket][0]

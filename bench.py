@@ -1,22 +1,30 @@
 """MVS throughput benchmark on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2]
-                    [--batched-step 1|2] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c3|c4]
+                    [--batched-step auto|1|2] [--no-cpu-baseline]
 
-Workload (default, config c2): a synthetic site of 10 views of 1024x1024
-uint8 (5-30 deg off-nadir, 8 azimuths, 600 km cameras); every view is a
-reference with 8 sources; photometric pass (5 iterations) then geometric
-pass with filter (5 iterations); then 2.5D fusion into a 0.5 m DSM.  One
-"step" solves all 10 reference views and fuses them, with the images already
-resident in HBM.
+Configurations (synthetic sites, seeded; images resident in HBM before
+timing; one JSON line on stdout):
 
-Two forms are timed on the same inputs:
-  plain    one reference view per vsm_solve call, synchronised per view;
-  batched  all views in one vsm_solve per pass, the half-step sequence
-           replayed from a captured hipGraph.
-`value` is the batched views/s; the plain figure, per-view latency
-percentiles and the DSM error against the synthetic ground truth of both
-forms are reported beside it.  One JSON line on stdout.
+c2 (default)  10 views of 1024x1024, every view a reference with 8 sources;
+              photometric pass (5 iterations), geometric pass + filter
+              (5 iterations), 0.5 m DSM fusion.  One step solves and fuses
+              all 10 views.  Two forms on the same inputs:
+                plain    one reference view per solve call, synchronised,
+                         window step 1;
+                batched  all views in one solve per pass, the half-step
+                         sequence replayed from a captured hipGraph, window
+                         step chosen by the autotuner.
+              `value` = batched views/s; plain views/s, per-view latency
+              percentiles and the DSM error of both forms vs the synthetic
+              ground truth are reported beside it.
+c3            32 reference views of ragged crop sizes (512..2048 per side),
+              6..12 sources each, same passes; plain vs batched.
+c4            serving: Poisson arrivals (default 10 req/s) of per-view depth
+              requests from 4 sites (crops 1024..2048 per side, 8..11
+              sources); continuous batching with HBM-footprint admission;
+              p50/p99 latency and views/s, against a one-request-at-a-time
+              queue on the same arrival schedule.
 """
 import argparse
 import json
@@ -39,27 +47,33 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def build_site(cfg, work):
-    from vissatsatellitestereo_amd import synth, workspace
+# ---------------------------------------------------------------------------
+# sites
+# ---------------------------------------------------------------------------
+def _device():
     import torch
-    dev = "cuda" if torch.cuda.is_available() else "cpu"
-    if cfg == "c2":
-        spec = synth.spec_c2()
-        n_src = 8
-    elif cfg == "c2s":       # reduced c2 for quick checks
-        spec = synth.spec_c2(n_views=6, size=512)
-        n_src = 5
-    else:
-        raise SystemExit(f"unknown config {cfg}")
+    return "cuda" if torch.cuda.is_available() else "cpu"
+
+
+def build_site(spec, work, max_src, src_counts=None):
+    """Render + dense workspace; src_counts: optional per-view source counts
+    written as explicit patch-match.cfg lists."""
+    from vissatsatellitestereo_amd import synth, workspace
     if not os.path.exists(os.path.join(work, "colmap", "mvs", "depth_ranges.txt")):
         t = time.time()
-        synth.build_site(spec, work, device=dev, save_gt_views=False, log=log)
-        workspace.prepare_dense_workspace(work, max_src=n_src)
-        log(f"site {cfg} built in {time.time() - t:.1f}s")
-    return spec, n_src
+        synth.build_site(spec, work, device=_device(), save_gt_views=False, log=log)
+        mvs = workspace.prepare_dense_workspace(work, max_src=max_src)
+        if src_counts is not None:
+            sel = workspace.select_sources(mvs)
+            names = sorted(sel)
+            with open(os.path.join(mvs, "stereo", "patch-match.cfg"), "w") as fp:
+                for n, k in zip(names, src_counts):
+                    fp.write(n + "\n" + ", ".join(sel[n][:k]) + "\n")
+        log(f"site {work} built in {time.time() - t:.1f}s")
+    return work
 
 
-def load_views(work):
+def load_site(work, id_base=0):
     from vissatsatellitestereo_amd import reparam_depth as rd
     from vissatsatellitestereo_amd.workspace import select_sources
     from PIL import Image
@@ -69,9 +83,9 @@ def load_views(work):
     imgs = {n: np.array(Image.open(os.path.join(mvs, "images", n))) for n in names}
     dr = rd.read_name_values(os.path.join(mvs, "depth_ranges.txt"))
     srcs = select_sources(mvs)
-    ids = {n: i + 1 for i, n in enumerate(names)}
+    ids = {n: id_base + i + 1 for i, n in enumerate(names)}
     views = [dict(ref=ids[n], srcs=[ids[s] for s in srcs[n]], depth_range=dr[n]) for n in names]
-    return names, imgs, P4, views, ids
+    return dict(work=work, names=names, imgs=imgs, P4=P4, views=views, ids=ids)
 
 
 def dsm_error(dsm, gt):
@@ -81,6 +95,9 @@ def dsm_error(dsm, gt):
     return float(np.median(np.abs(dsm[ok] - gt[ok]))), float(ok.mean())
 
 
+# ---------------------------------------------------------------------------
+# plain vs batched
+# ---------------------------------------------------------------------------
 def run_form(ctx, views, grid, form, iters, use_graph):
     """One step: photometric + geometric for every view, then fusion.
     Returns (wall seconds, per-view latencies, fused DSM)."""
@@ -103,84 +120,10 @@ def run_form(ctx, views, grid, form, iters, use_graph):
     return wall, list(lat.values()), dsm
 
 
-def cpu_baseline(names, imgs, P4, views, ids, window_radius, seconds_hint=20.0):
-    """Oracle (FP64 C, OpenMP) on a bounded sample: a 512x512 crop of the
-    first reference view with all its sources, 1 photometric iteration;
-    extrapolated to a full view (area x iterations)."""
-    import pm_oracle as po
-    v = views[0]
-    inv_ids = {i: n for n, i in ids.items()}
-    ref = inv_ids[v["ref"]]
-    srcs = [inv_ids[s] for s in v["srcs"]]
-    H, W = imgs[ref].shape
-    ch, cw = min(512, H), min(512, W)
-    y0, x0 = (H - ch) // 2, (W - cw) // 2
-    crop = np.ascontiguousarray(imgs[ref][y0:y0 + ch, x0:x0 + cw])
-    T = np.array([[1, 0, x0, 0], [0, 1, y0, 0], [0, 0, 1, 0], [0, 0, 0, 1]], dtype=np.float64)
-    # P4 of the crop: pixel (c, r) of the crop is (c + x0, r + y0) of the view
-    P4c = np.linalg.inv(T) @ P4[ref]
-    ov = po.OracleView(crop, [imgs[s] for s in srcs], P4c, [P4[s] for s in srcs], v["depth_range"],
-                       view_id=v["ref"], params=dict(window_radius=window_radius))
-    t = time.perf_counter()
-    ov.solve(0, 1)
-    dt = time.perf_counter() - t
-    scale = (H * W) / (ch * cw)
-    return dt, scale, f"{ch}x{cw} crop of one {W}x{H} reference view, {len(srcs)} sources, " \
-                      f"1 photometric iteration ({dt:.1f}s); per-view time = sample x {scale:.0f} " \
-                      f"(area) x iterations"
-
-
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2")
-    ap.add_argument("--iters", type=int, default=5)
-    ap.add_argument("--window-radius", type=int, default=5)
-    ap.add_argument("--batched-step", type=int, default=1, help="window stride of the batched form")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--forms", default="plain,batched", help="comma list of plain,batched")
-    ap.add_argument("--build-only", action="store_true", help="build the synthetic site and exit")
-    ap.add_argument("--copy-calibration-mb", type=int, default=0,
-                    help="run a device copy of this many MiB first (HBM counter calibration)")
-    ap.add_argument("--work", default=os.environ.get("VSM_BENCH_WORK", "/tmp/vsm_bench"))
-    a = ap.parse_args()
-    if a.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
-                          "n_gpus": a.gpus}))
-        sys.exit(2)
-
-    from vissatsatellitestereo_amd.mvs import MvsContext
-    work = os.path.join(a.work, a.config)
-    spec, n_src = build_site(a.config, work)
-    if a.build_only:
-        return
-    if a.copy_calibration_mb > 0:
-        import torch
-        n = a.copy_calibration_mb * (1 << 20) // 4
-        src = torch.ones(n, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"copy calibration: {a.copy_calibration_mb} MiB read + written")
-        del src, dst
-    names, imgs, P4, views, ids = load_views(work)
-    gt = np.load(os.path.join(work, "synthetic_gt", "gt_dsm.npy"))
-    from vissatsatellitestereo_amd import runtime
-    grid = runtime.dsm_grid(runtime.read_aoi(work))
-
-    ctx = MvsContext(device=0, window_radius=a.window_radius, window_step=1, num_samples=15)
-    for n in names:
-        ctx.add_image(ids[n], imgs[n], P4[n])
-    ctx.synchronize()
-    H, W = imgs[names[0]].shape
-    nviews = len(views)
-
+def measure_forms(ctx, views, grid, gt, a, batched_step):
     res = {}
-    forms = [f for f in ("plain", "batched") if f in a.forms.split(",")]
-    for form in forms:
-        ctx.set_params(window_step=1 if form == "plain" else a.batched_step)
+    for form in [f for f in ("plain", "batched") if f in a.forms.split(",")]:
+        ctx.set_params(window_step=1 if form == "plain" else batched_step)
         use_graph = form == "batched"
         for _ in range(a.warmup):
             run_form(ctx, views, grid, form, a.iters, use_graph)
@@ -188,7 +131,6 @@ def main():
         ctx.reset_stats()
         ctx.set_timing(form == "batched")
         walls, lats = [], []
-        dsm = None
         ctx.synchronize()
         for k in range(a.steps):
             w, lat, dsm = run_form(ctx, views, grid, form, a.iters, use_graph)
@@ -199,48 +141,121 @@ def main():
         st = ctx.stats()
         ctx.set_timing(False)
         err, cov = dsm_error(dsm, gt)
-        res[form] = dict(wall=float(np.sum(walls)), ms_per_step=1e3 * float(np.mean(walls)),
-                         views_per_s=nviews * a.steps / float(np.sum(walls)),
+        res[form] = dict(ms_per_step=1e3 * float(np.mean(walls)),
+                         views_per_s=len(views) * a.steps / float(np.sum(walls)),
                          p50_ms=1e3 * float(np.percentile(lats, 50)),
                          p99_ms=1e3 * float(np.percentile(lats, 99)),
                          dsm_err=err, dsm_cov=cov, stats=st)
         log(form, {k: v for k, v in res[form].items() if k != "stats"})
+    return res
 
+
+def roofline(st):
+    if not (st["halfstep_ms"] > 0 and st["halfstep_launches"] > 0):
+        return None
+    avg_ms = st["halfstep_ms"] / st["halfstep_launches"]
+    taps = st["taps"] / st["halfstep_launches"]
+    ach = FLOPS_PER_TAP * taps / (avg_ms * 1e-3) / 1e12
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "k_halfstep_c2_counters.json")
+    if os.path.exists(tfile):
+        t = json.load(open(tfile))
+        traffic = {"bytes_per_launch": round(t["fetch_bytes"] + t["write_bytes"]),
+                   "source": "profiles/k_halfstep_c2_counters.md (FETCH_SIZE + WRITE_SIZE, "
+                             "c2 batched, window step 1)"}
+    return {"bound": "valu", "kernel": "k_halfstep", "achieved": round(ach, 3),
+            "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / FP32_PEAK_TFLOPS, 4),
+            "traffic": traffic, "avg_launch_ms": round(avg_ms, 4), "taps_per_launch": int(taps),
+            "flops_per_tap": FLOPS_PER_TAP}
+
+
+def cpu_baseline(site, window_radius, iters):
+    """Oracle (FP64 C, OpenMP) on a bounded sample: a 512x512 crop of the
+    first reference view with all its sources, 1 photometric iteration,
+    extrapolated to a full view (area x iterations of both passes)."""
+    import pm_oracle as po
+    v = site["views"][0]
+    inv_ids = {i: n for n, i in site["ids"].items()}
+    ref = inv_ids[v["ref"]]
+    srcs = [inv_ids[s] for s in v["srcs"]]
+    imgs, P4 = site["imgs"], site["P4"]
+    H, W = imgs[ref].shape
+    ch, cw = min(512, H), min(512, W)
+    y0, x0 = (H - ch) // 2, (W - cw) // 2
+    crop = np.ascontiguousarray(imgs[ref][y0:y0 + ch, x0:x0 + cw])
+    T = np.array([[1, 0, x0, 0], [0, 1, y0, 0], [0, 0, 1, 0], [0, 0, 0, 1]], dtype=np.float64)
+    P4c = np.linalg.inv(T) @ P4[ref]          # pixel (c, r) of the crop is (c + x0, r + y0)
+    ov = po.OracleView(crop, [imgs[s] for s in srcs], P4c, [P4[s] for s in srcs], v["depth_range"],
+                       view_id=v["ref"], params=dict(window_radius=window_radius))
+    t = time.perf_counter()
+    ov.solve(0, 1)
+    dt = time.perf_counter() - t
+    scale = (H * W) / (ch * cw)
+    per_view = dt * scale * (2 * iters)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(1.0 / per_view, 6), "unit": "views/s", "cores": cores, "kind": "port",
+            "sample": f"{ch}x{cw} crop of one {W}x{H} reference view, {len(srcs)} sources, "
+                      f"1 photometric iteration of the FP64 oracle in {dt:.1f}s; per-view time = "
+                      f"sample x {scale:.0f} (area) x {2 * iters} (iterations of both passes)"}
+
+
+def make_ctx(sites, a):
+    from vissatsatellitestereo_amd.mvs import MvsContext
+    ctx = MvsContext(device=0, hbm_budget_bytes=int(a.hbm_budget_gb * 1e9),
+                     window_radius=a.window_radius, window_step=1, num_samples=15)
+    for s in sites:
+        for n in s["names"]:
+            ctx.add_image(s["ids"][n], s["imgs"][n], s["P4"][n])
+    ctx.synchronize()
+    return ctx
+
+
+def choose_step(ctx, views, a):
+    if a.batched_step != "auto":
+        return int(a.batched_step), {"window_step": int(a.batched_step), "source": "command line"}
+    from vissatsatellitestereo_amd import autotune
+    st, rep = autotune.tune(ctx, views, iters=a.iters, radius=a.window_radius,
+                            use_cache=not a.no_tune_cache)
+    log("autotune", rep)
+    return st, rep
+
+
+def bench_forms(a, cfg):
+    from vissatsatellitestereo_amd import synth, runtime
+    if cfg == "c2":
+        spec, max_src, counts = synth.spec_c2(), 8, None
+        desc = "10 reference views x 1024x1024 uint8, 8 sources"
+    else:
+        spec = synth.spec_ragged(32)
+        rng = np.random.default_rng(7)
+        counts = [int(k) for k in rng.integers(6, 13, spec.n_views)]
+        max_src = 12
+        desc = "32 reference views, ragged crops 512..2048 per side, 6..12 sources"
+    work = build_site(spec, os.path.join(a.work, cfg), max_src, counts)
+    if a.build_only:
+        return None
+    site = load_site(work)
+    gt = np.load(os.path.join(work, "synthetic_gt", "gt_dsm.npy"))
+    grid = runtime.dsm_grid(runtime.read_aoi(work))
+    copy_calibration(a)
+    ctx = make_ctx([site], a)
+    views = site["views"]
+    step, tune_rep = choose_step(ctx, views, a)
+    res = measure_forms(ctx, views, grid, gt, a, step)
     b = res.get("batched") or res["plain"]
-    st = b["stats"]
-    roof = None
-    if st["halfstep_ms"] > 0 and st["halfstep_launches"] > 0:
-        avg_ms = st["halfstep_ms"] / st["halfstep_launches"]
-        taps_per_launch = st["taps"] / st["halfstep_launches"]
-        ach = FLOPS_PER_TAP * taps_per_launch / (avg_ms * 1e-3) / 1e12
-        roof = {"bound": "valu", "kernel": "k_halfstep", "achieved": round(ach, 3),
-                "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / FP32_PEAK_TFLOPS, 4),
-                "traffic": None, "avg_launch_ms": round(avg_ms, 4),
-                "taps_per_launch": int(taps_per_launch), "flops_per_tap": FLOPS_PER_TAP}
-
-    cpu = None
-    if not a.no_cpu_baseline:
-        try:
-            dt, scale, sample = cpu_baseline(names, imgs, P4, views, ids, a.window_radius)
-            per_view = dt * scale * (2 * a.iters)
-            cpu = {"value": round(1.0 / per_view, 6), "unit": "views/s",
-                   "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
-                   "kind": "port", "sample": sample}
-        except Exception as e:      # the baseline is reported, never required
-            cpu = {"value": None, "unit": "views/s", "cores": None, "kind": "port",
-                   "sample": f"failed: {e}"}
-
     p = res.get("plain") or b
+    cpu = None if a.no_cpu_baseline else cpu_baseline(site, a.window_radius, a.iters)
     out = {
         "metric": METRIC, "value": round(b["views_per_s"], 4), "unit": "views/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(b["ms_per_step"], 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": f"{a.config}: {nviews} reference views x {W}x{H} uint8, {n_src} sources, "
-                               f"photometric+geometric {a.iters}+{a.iters} iterations, filter, "
-                               f"r={a.window_radius}, 0.5 m DSM fusion",
-                   "forms": {"plain": "one view per solve, window step 1",
-                             "batched": f"all views per solve, hipGraph replay, window step {a.batched_step}"}},
+        "config": {"workload": f"{cfg}: {desc}, photometric+geometric {a.iters}+{a.iters} "
+                               f"iterations, filter, r={a.window_radius}, 0.5 m DSM fusion",
+                   "forms": {"plain": "one view per solve call, window step 1",
+                             "batched": f"all views per solve call, persistent kernel, hipGraph "
+                                        f"replay, window step {step} (autotuner)"},
+                   "autotune": tune_rep},
         "plain_views_per_s": round(p["views_per_s"], 4),
         "batched_over_plain": round(b["views_per_s"] / p["views_per_s"], 3),
         "latency_ms": {"plain_p50": round(p["p50_ms"], 1), "plain_p99": round(p["p99_ms"], 1),
@@ -248,11 +263,122 @@ def main():
         "dsm_median_abs_err_m": {"plain": p["dsm_err"], "batched": b["dsm_err"],
                                  "coverage_plain": round(p["dsm_cov"], 4),
                                  "coverage_batched": round(b["dsm_cov"], 4)},
-        "roofline": roof,
+        "roofline": roofline(b["stats"]),
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
     ctx.close()
+    return out
+
+
+# ---------------------------------------------------------------------------
+# serving
+# ---------------------------------------------------------------------------
+def bench_serving(a):
+    from vissatsatellitestereo_amd import synth
+    from vissatsatellitestereo_amd.serving import ServeLoop, poisson_arrivals
+    works = []
+    for k in range(4):
+        spec = synth.spec_ragged(n_views=12, lo=1024, hi=a.serve_max_side, seed=50 + k, n_boxes=30,
+                                 seed_offset=10 * (k + 1))
+        rng = np.random.default_rng(60 + k)
+        counts = [int(c) for c in rng.integers(8, 12, spec.n_views)]
+        works.append(build_site(spec, os.path.join(a.work, f"c4_site{k}"), 11, counts))
+    if a.build_only:
+        return None
+    sites = [load_site(w, id_base=1000 * (k + 1)) for k, w in enumerate(works)]
+    copy_calibration(a)
+    ctx = make_ctx(sites, a)
+    ctx.set_params(num_iterations=a.iters, geom_consistency=0, filter=1,
+                   window_step=1 if a.batched_step == "auto" else int(a.batched_step))
+    views = [v for s in sites for v in s["views"]]
+    rng = np.random.default_rng(6)
+    order = rng.permutation(len(views))
+    reqs = [views[i] for i in order][:a.requests]
+    arrivals = poisson_arrivals(len(reqs), a.rate, seed=6)
+    warm = ServeLoop(ctx)                    # one request end to end
+    warm.submit(reqs[0])
+    warm.drain()
+    loop = ServeLoop(ctx)
+    ctx.reset_stats()
+    served = loop.run_schedule(arrivals, reqs)
+    st = ctx.stats()
+    log("served", {k: v for k, v in served.items() if k not in ("tickets", "latency_s")})
+    # one-request-at-a-time queue on the same arrival schedule
+    t0 = time.perf_counter()
+    lat = []
+    for t_arr, v in zip(arrivals, reqs):
+        now = time.perf_counter() - t0
+        if now < t_arr:
+            time.sleep(t_arr - now)
+        ctx.solve([v], 0, want=("depth",))
+        lat.append(time.perf_counter() - t0 - t_arr)
+    plain_wall = time.perf_counter() - t0
+    log(f"queue: {len(reqs) / plain_wall:.2f} views/s")
+    out = {
+        "metric": METRIC, "value": round(served["views_per_s"], 4), "unit": "views/s", "n_gpus": 1,
+        "steps": len(reqs), "warmup": 1, "ms_per_step": round(1e3 * served["wall_s"] / len(reqs), 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"c4 serving: {len(reqs)} per-view requests, Poisson {a.rate} req/s, "
+                               f"4 sites, crops 1024..{a.serve_max_side} per side, 8..11 sources, "
+                               f"photometric {a.iters} iterations + filter, r={a.window_radius}, "
+                               f"HBM budget {a.hbm_budget_gb} GB"},
+        "latency_ms": {"served_p50": round(served["p50_ms"], 1), "served_p99": round(served["p99_ms"], 1),
+                       "queue_p50": round(1e3 * float(np.percentile(lat, 50)), 1),
+                       "queue_p99": round(1e3 * float(np.percentile(lat, 99)), 1)},
+        "plain_views_per_s": round(len(reqs) / plain_wall, 4),
+        "batched_over_plain": round(served["views_per_s"] / (len(reqs) / plain_wall), 3),
+        "serve_steps": served["steps"],
+        "roofline": roofline(st),
+        "cpu_baseline": None,
+    }
+    ctx.close()
+    return out
+
+
+def copy_calibration(a):
+    """Optional: a device copy of known size (HBM counter calibration); a
+    512 MiB flush buffer written in between pushes the source out of L2/MALL."""
+    if a.copy_calibration_mb <= 0:
+        return
+    import torch
+    n = a.copy_calibration_mb * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    flush = torch.ones(max(n, (512 << 20) // 4), dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copy calibration: {a.copy_calibration_mb} MiB read + written")
+    del src, dst, flush
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2", choices=("c2", "c3", "c4"))
+    ap.add_argument("--iters", type=int, default=5)
+    ap.add_argument("--window-radius", type=int, default=5)
+    ap.add_argument("--batched-step", default="auto", help="auto (autotuner), 1 or 2")
+    ap.add_argument("--no-tune-cache", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--forms", default="plain,batched", help="comma list of plain,batched")
+    ap.add_argument("--build-only", action="store_true", help="build the synthetic site(s) and exit")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0)
+    ap.add_argument("--hbm-budget-gb", type=float, default=200.0)
+    ap.add_argument("--rate", type=float, default=10.0, help="c4 arrival rate (req/s)")
+    ap.add_argument("--requests", type=int, default=40, help="c4 number of requests")
+    ap.add_argument("--serve-max-side", type=int, default=2048)
+    ap.add_argument("--work", default=os.environ.get("VSM_BENCH_WORK", "/tmp/vsm_bench"))
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; runs on more GPUs are not part of it",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+    out = bench_serving(a) if a.config == "c4" else bench_forms(a, a.config)
+    if out is not None:
+        print(json.dumps(out))
 
 
 if __name__ == "__main__":

@@ -95,7 +95,8 @@ def test_autotune_picks_and_caches(tiny, tmp_path):
     cache = str(tmp_path / "tune.json")
     st, rep = autotune.tune(ctx, views, iters=3, radius=4, cache_path=cache)
     print("autotune", rep)
-    assert st in (1, 2) and rep["agreement"]["1"] == 1.0 and not rep["cached"]
+    assert st in (1, 2) and not rep["cached"] and rep["calib_views"] == 4
+    assert rep["kept_fraction"][str(st)] >= rep["kept_fraction"]["1"] - rep["max_drop"]
     st2, rep2 = autotune.tune(ctx, views, iters=3, radius=4, cache_path=cache)
     assert st2 == st and rep2["cached"]
     ctx.close()

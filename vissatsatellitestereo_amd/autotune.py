@@ -1,13 +1,17 @@
 """Autotuner for the batched form's window schedule.
 
 Knob: the tap stride inside the matching window (``window_step`` 1 or 2;
-stride 2 visits (floor(2r/2)+1)^2 taps instead of (2r+1)^2).  A candidate is
-accepted when the depth maps it produces on a calibration batch agree with
-the stride-1 maps: at least ``min_agreement`` of the pixels that stride 1
-matched confidently (cost < 0.5) lie within ``depth_tol`` of the stride-1
-depth.  Among accepted candidates the fastest by device time wins.  Results
-are cached in a JSON file keyed by (height, width, sources, radius,
-iterations) size classes.
+stride 2 visits (floor(2r/2)+1)^2 taps instead of (2r+1)^2).
+
+Accuracy proxy (no ground truth needed): multi-view geometric consistency.
+Each candidate solves a calibration batch (up to ``calib_views`` reference
+views, sources restricted to that batch) through the photometric pass, the
+geometric pass and the filter; the fraction of pixels the filter keeps is the
+share of depths that at least two sources confirm photometrically and by
+forward-backward reprojection.  A candidate is accepted when its kept
+fraction is within ``max_drop`` of stride 1's; the fastest accepted one (wall
+time of both passes) wins.  Results are cached in a JSON file keyed by
+(height, width, sources, radius, iterations) size classes.
 """
 import json
 import os
@@ -44,44 +48,48 @@ def save_cache(cache, path=CACHE):
     os.replace(tmp, path)
 
 
-def tune(ctx, views, iters, radius, steps=(1, 2), min_agreement=0.9, depth_tol=0.5,
-         calib_views=2, cache_path=CACHE, use_cache=True):
-    """Pick window_step for `views` on `ctx`; returns (step, report)."""
+def tune(ctx, views, iters, radius, steps=(1, 2), max_drop=0.02, calib_views=8,
+         cache_path=CACHE, use_cache=True):
+    """Pick window_step for `views` on `ctx`; returns (step, report).
+
+    Note: the calibration solves replace the context's stored photometric
+    and geometric maps of the calibration views."""
     shape = ctx.sizes[int(views[0]["ref"])]
     n_src = max(len(v["srcs"]) for v in views)
     key = key_of(shape, n_src, radius, iters)
     cache = load_cache(cache_path) if use_cache else {}
     if key in cache:
         return cache[key]["window_step"], dict(cache[key], cached=True)
-    calib = views[:calib_views]
+    ids = {int(v["ref"]) for v in views[:calib_views]}
+    calib = []
+    for v in views[:calib_views]:
+        srcs = [s for s in v["srcs"] if int(s) in ids]
+        if srcs:
+            calib.append(dict(v, srcs=srcs))
     saved = ctx.params
-    ctx.set_params(window_radius=radius, num_iterations=iters, geom_consistency=0, filter=0)
-    results, times = {}, {}
+    kept, times = {}, {}
     try:
         for st in steps:
-            ctx.set_params(window_step=st)
-            ctx.solve(calib, 0, want=("depth",))            # warm-up (code objects, allocations)
+            ctx.set_params(window_radius=radius, num_iterations=iters, window_step=st,
+                           geom_consistency=0, filter=0)
+            ctx.solve(calib[:1], 0, want=())                     # warm-up
             ctx.synchronize()
             t = time.perf_counter()
-            results[st] = ctx.solve(calib, 0, want=("depth", "cost"))
+            ctx.solve(calib, 0, want=())
+            ctx.set_params(geom_consistency=1, filter=1)
+            res = ctx.solve(calib, 1, want=("depth",))
             ctx.synchronize()
             times[st] = time.perf_counter() - t
+            kept[st] = float(np.mean([np.mean(r["depth"] > 0) for r in res]))
     finally:
-        ctx.set_params(window_step=saved.window_step, num_iterations=saved.num_iterations,
+        ctx.set_params(window_radius=saved.window_radius, window_step=saved.window_step,
+                       num_iterations=saved.num_iterations,
                        geom_consistency=saved.geom_consistency, filter=saved.filter)
-    ref = results[steps[0]]
-    agree = {}
-    for st in steps:
-        fr = []
-        for a, b in zip(ref, results[st]):
-            conf = a["cost"] < 0.5
-            fr.append(float(np.mean(np.abs(a["depth"] - b["depth"])[conf] < depth_tol)) if conf.any() else 1.0)
-        agree[st] = float(np.mean(fr))
-    ok = [st for st in steps if agree[st] >= min_agreement]
-    best = min(ok, key=lambda st: times[st]) if ok else steps[0]
+    ok = [st for st in steps if kept[st] >= kept[steps[0]] - max_drop]
+    best = min(ok, key=lambda st: times[st])
     rep = dict(window_step=best, key=key, seconds={str(k): v for k, v in times.items()},
-               agreement={str(k): v for k, v in agree.items()}, min_agreement=min_agreement,
-               depth_tol=depth_tol, cached=False)
+               kept_fraction={str(k): v for k, v in kept.items()}, max_drop=max_drop,
+               calib_views=len(calib), cached=False)
     if use_cache:
         cache[key] = rep
         save_cache(cache, cache_path)

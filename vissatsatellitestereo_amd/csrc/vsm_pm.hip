@@ -29,6 +29,7 @@ struct SrcLds {
     float A[12];                 // tile-centred ref -> src (anchor-relative), normalised
     float B[12];                 // src (anchor-relative) -> ref (tile-centre-relative)
     int ax, ay, W, H;
+    int base;                    // ay * W + ax: texel offset of the anchor
     float lox, hix, loy, hiy;    // in-image box in anchor-relative coordinates
     const uint32_t *quad;
     const float *depth;
@@ -78,6 +79,7 @@ __device__ void setup_source(const ViewDesc &V, int s, int uc, int vc, SrcLds &o
     for (int k = 0; k < 12; ++k) { o.A[k] = (float)(A[k] * ma); o.B[k] = (float)(B[k] * mb); }
     o.ax = ax; o.ay = ay;
     o.W = V.src_w[s]; o.H = V.src_h[s];
+    o.base = ay * o.W + ax;
     o.lox = (float)(-ax); o.hix = (float)(o.W - 1 - ax);
     o.loy = (float)(-ay); o.hiy = (float)(o.H - 1 - ay);
     o.quad = V.src_quad[s];
@@ -105,14 +107,24 @@ __device__ __forceinline__ Proj make_proj(const SrcLds &S, float du, float dv, f
 typedef const __attribute__((address_space(1))) uint32_t *gquad_t;
 typedef const __attribute__((address_space(1))) float *gfloat_t;
 
-// Bilinear sample (DN) at anchor-relative (px, py), already clamped in-image.
-// The texel index fits 24 bits x 16 bits (images <= 32768 x 32768, <= 2^30 px).
+// floor(x) as an integer in one instruction (x finite).
+__device__ __forceinline__ int cvt_floor(float x)
+{
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+// Bilinear sample (DN) at anchor-relative (px, py), already inside the image.
+// The texel index iy*W + ix is formed with a signed 24-bit multiply-add on
+// anchor-relative integers plus the anchor's offset ay*W + ax (|relative
+// coordinates| < 2^23, image side <= 32768).
 __device__ __forceinline__ float sample_quad(const SrcLds &S, float px, float py)
 {
-    const float fx0 = floorf(px), fy0 = floorf(py);
-    const float fx = px - fx0, fy = py - fy0;
-    const unsigned ix = (unsigned)(S.ax + (int)fx0), iy = (unsigned)(S.ay + (int)fy0);
-    const uint32_t v = ((gquad_t)S.quad)[__umul24(iy, (unsigned)S.W) + ix];
+    const int ix = cvt_floor(px), iy = cvt_floor(py);
+    const float fx = __builtin_amdgcn_fractf(px), fy = __builtin_amdgcn_fractf(py);
+    const int idx = __mul24(iy, S.W) + (ix + S.base);
+    const uint32_t v = ((gquad_t)S.quad)[(unsigned)idx];
     const float a = (float)(v & 0xFFu), b = (float)((v >> 8) & 0xFFu);
     const float c = (float)((v >> 16) & 0xFFu), d = (float)(v >> 24);
     const float top = fmaf(fx, b - a, a);
@@ -128,7 +140,7 @@ __device__ __forceinline__ float tap_weight(const PMConst &c, float r, int dx, i
 }
 
 struct RefStats {
-    float rc, sw, mr, var_r;
+    float rc, sw, inv_sw, mr, var_r;
 };
 
 __device__ __forceinline__ RefStats ref_stats(const PMConst &c, const float *s_ref, int lbase)
@@ -149,8 +161,9 @@ __device__ __forceinline__ RefStats ref_stats(const PMConst &c, const float *s_r
         }
     }
     st.sw = sw;
-    st.mr = sr / sw;
-    st.var_r = srr / sw - st.mr * st.mr;
+    st.inv_sw = 1.f / sw;
+    st.mr = sr * st.inv_sw;
+    st.var_r = srr * st.inv_sw - st.mr * st.mr;
     return st;
 }
 
@@ -164,11 +177,11 @@ constexpr float kDeltaMax = 1e-3f;
 // weight and the reference value of each tap are computed once and shared;
 // the G gathers per tap are independent.  EXACT selects the per-tap
 // reciprocal (fallback when some lane's window exceeds kDeltaMax).
-template <int G, bool EXACT>
+template <int G, bool EXACT, bool CLAMP>
 __device__ __forceinline__ void window_pass(const PMConst &c, const SrcLds &S, const Proj (&P)[G],
-                                            const float (&iz0)[G], const float (&sc)[G],
-                                            const float *s_ref, int lbase, const RefStats &R,
-                                            float (&ss)[G], float (&sss)[G], float (&srs)[G])
+                                            const float (&iz0)[G], const float *s_ref, int lbase,
+                                            const RefStats &R, float (&ss)[G], float (&sss)[G],
+                                            float (&srs)[G])
 {
     const int rad = c.radius, st = c.step, nt = c.ntaps;
     const float frad = (float)(-rad);
@@ -204,10 +217,11 @@ __device__ __forceinline__ void window_pass(const PMConst &c, const SrcLds &S, c
                 const float qy = fmaf(fi, P[k].huy, ry[k]);
                 const float iz = EXACT ? __builtin_amdgcn_rcpf(fmaf(fi, P[k].huz, rz[k]))
                                        : fmaf(fi, giu[k], rz[k]);
-                // hypotheses are finite (NaN-free), so med3 is a plain clamp
-                const float px = __builtin_amdgcn_fmed3f(qx * iz, S.lox, S.hix);
-                const float py = __builtin_amdgcn_fmed3f(qy * iz, S.loy, S.hiy);
-                const float sv = sample_quad(S, px, py) - sc[k];
+                // hypotheses are finite (NaN-free), so med3 is a plain clamp;
+                // windows proven inside the image skip it
+                const float px = CLAMP ? __builtin_amdgcn_fmed3f(qx * iz, S.lox, S.hix) : qx * iz;
+                const float py = CLAMP ? __builtin_amdgcn_fmed3f(qy * iz, S.loy, S.hiy) : qy * iz;
+                const float sv = sample_quad(S, px, py) - R.rc;
                 const float ws = w * sv;
                 ss[k] += ws;
                 sss[k] = fmaf(ws, sv, sss[k]);
@@ -217,39 +231,46 @@ __device__ __forceinline__ void window_pass(const PMConst &c, const SrcLds &S, c
     }
 }
 
-// NCC cost of G plane hypotheses against source S (same arithmetic per
-// hypothesis as ncc_cost up to the reciprocal of the window pass).
+// NCC cost of G plane hypotheses against source S.  Source samples are
+// centred on the reference centre value rc before accumulation (NCC is
+// invariant to the offset; it only keeps the FP32 sums well conditioned).
 template <int G>
 __device__ __forceinline__ void ncc_cost_group(const PMConst &c, const SrcLds &S, const Proj (&P)[G],
                                                const float *s_ref, int lbase, const RefStats &R,
                                                float (&cost)[G])
 {
     bool valid[G];
-    float sc[G], iz0[G], ss[G], sss[G], srs[G];
-    bool far = false;
+    float iz0[G], ss[G], sss[G], srs[G];
+    bool far = false, edge = false;
     const float fr = (float)c.radius;
 #pragma unroll
     for (int k = 0; k < G; ++k) {
         valid[k] = P[k].q0z > 0.f;
-        iz0[k] = valid[k] ? 1.f / P[k].q0z : 1.f;
+        iz0[k] = valid[k] ? __builtin_amdgcn_rcpf(P[k].q0z) : 1.f;
         const float cx = P[k].q0x * iz0[k], cy = P[k].q0y * iz0[k];
         valid[k] = valid[k] && cx >= S.lox && cx <= S.hix && cy >= S.loy && cy <= S.hiy;
-        sc[k] = sample_quad(S, __builtin_amdgcn_fmed3f(cx, S.lox, S.hix),
-                            __builtin_amdgcn_fmed3f(cy, S.loy, S.hiy));
         const float delta = fr * (fabsf(P[k].huz) + fabsf(P[k].hvz)) * fabsf(iz0[k]);
         far = far || !(delta <= kDeltaMax);
+        // conservative half-extent of the projected window (+1 px guard)
+        const float jxx = (P[k].hux - cx * P[k].huz) * iz0[k], jxy = (P[k].hvx - cx * P[k].hvz) * iz0[k];
+        const float jyx = (P[k].huy - cy * P[k].huz) * iz0[k], jyy = (P[k].hvy - cy * P[k].hvz) * iz0[k];
+        const float mx = fr * (fabsf(jxx) + fabsf(jxy)) * (1.f + 2.f * delta) + 1.f;
+        const float my = fr * (fabsf(jyx) + fabsf(jyy)) * (1.f + 2.f * delta) + 1.f;
+        edge = edge || !(cx - mx >= S.lox && cx + mx <= S.hix && cy - my >= S.loy && cy + my <= S.hiy);
     }
     if (__ballot(far) != 0ull)
-        window_pass<G, true>(c, S, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
+        window_pass<G, true, true>(c, S, P, iz0, s_ref, lbase, R, ss, sss, srs);
+    else if (__ballot(edge) != 0ull)
+        window_pass<G, false, true>(c, S, P, iz0, s_ref, lbase, R, ss, sss, srs);
     else
-        window_pass<G, false>(c, S, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
+        window_pass<G, false, false>(c, S, P, iz0, s_ref, lbase, R, ss, sss, srs);
 #pragma unroll
     for (int k = 0; k < G; ++k) {
-        const float ms = ss[k] / R.sw;
-        const float var_s = sss[k] / R.sw - ms * ms;
-        const float cov = srs[k] / R.sw - R.mr * ms;
+        const float ms = ss[k] * R.inv_sw;
+        const float var_s = sss[k] * R.inv_sw - ms * ms;
+        const float cov = srs[k] * R.inv_sw - R.mr * ms;
         const bool flat = R.var_r < c.min_var || var_s < c.min_var;
-        const float ncc = cov / sqrtf(R.var_r * var_s);
+        const float ncc = cov * __builtin_amdgcn_rsqf(R.var_r * var_s);
         cost[k] = (!valid[k] || flat) ? kMaxCost : fminf(fmaxf(1.f - ncc, 0.f), kMaxCost);
     }
 }

@@ -458,17 +458,21 @@ def _project(cam: Camera, X: np.ndarray):
 
 
 def sparse_points(scene: Scene, hits: list, rng) -> tuple:
-    """Surface points sampled in view 0 with visibility-checked tracks."""
+    """Surface points sampled in the smallest view (every other view covers
+    its footprint) with visibility-checked tracks."""
     cams = scene.cameras
     n = scene.spec.n_points
-    c0 = cams[0]
+    v0 = int(np.argmin([c.width * c.height for c in cams]))
+    c0 = cams[v0]
     rows = rng.integers(0, c0.height, n)
     cols = rng.integers(0, c0.width, n)
-    X = hits[0][rows, cols]
+    X = hits[v0][rows, cols]
     tracks = []
     for i in range(n):
-        tracks.append([(0, float(cols[i]), float(rows[i]))])
-    for j, cam in enumerate(cams[1:], start=1):
+        tracks.append([(v0, float(cols[i]), float(rows[i]))])
+    for j, cam in enumerate(cams):
+        if j == v0:
+            continue
         u, v, w = _project(cam, X)
         ui = np.round(u).astype(np.int64)
         vi = np.round(v).astype(np.int64)
@@ -578,3 +582,15 @@ def spec_c2(n_views=10, size=1024) -> SceneSpec:
     return SceneSpec(width=size, height=size, n_views=n_views, off_nadir=(5.0, 30.0),
                      azimuths=tuple(45.0 * k for k in range(8)), n_boxes=20,
                      n_points=3000)
+
+
+def spec_ragged(n_views=32, lo=512, hi=2048, seed=5, n_boxes=40, seed_offset=0) -> SceneSpec:
+    """n_views crops with width and height drawn independently and uniformly
+    from [lo, hi] (seeded), 5-30 deg off-nadir at 8 azimuths."""
+    rng = np.random.default_rng(seed)
+    sizes = [(int(rng.integers(lo, hi + 1)), int(rng.integers(lo, hi + 1))) for _ in range(n_views)]
+    return SceneSpec(n_views=n_views, sizes=sizes, off_nadir=(5.0, 30.0),
+                     azimuths=tuple(45.0 * k for k in range(8)), n_boxes=n_boxes, n_points=3000,
+                     seed_terrain=0 + seed_offset, seed_albedo=1 + seed_offset,
+                     seed_noise=2 + seed_offset, seed_boxes=3 + seed_offset,
+                     seed_cameras=4 + seed_offset)

@@ -77,24 +77,24 @@ def select_sources(mvs_dir, min_triangulation_angle=5.0):
     """Resolve ``patch-match.cfg`` into {ref_name: [src_name, ...]}."""
     _, images, points3D = read_model(os.path.join(mvs_dir, "sparse"), ext=".txt")
     by_name = {im.name: iid for iid, im in images.items()}
-    centers = camera_centers(images)
     cfg = parse_patch_match_cfg(os.path.join(mvs_dir, "stereo", "patch-match.cfg"))
-    # shared points and their triangulation angles per image pair
-    shared = {}
-    for p in points3D.values():
-        ids = [int(i) for i in p.image_ids]
-        for a in ids:
-            for b in ids:
-                if a != b:
-                    na, nb = images[a].name, images[b].name
-                    ra = centers[na] - p.xyz
-                    rb = centers[nb] - p.xyz
-                    c = ra @ rb / (np.linalg.norm(ra) * np.linalg.norm(rb))
-                    shared.setdefault((na, nb), []).append(np.degrees(np.arccos(np.clip(c, -1, 1))))
+    order = sorted(images)
+    row = {iid: k for k, iid in enumerate(order)}
+    pts = list(points3D.values())
+    X = np.array([p.xyz for p in pts], dtype=np.float64).reshape(-1, 3)
+    vis = np.zeros((len(order), len(pts)), dtype=bool)
+    for j, p in enumerate(pts):
+        for i in p.image_ids:
+            vis[row[int(i)], j] = True
+    centers = camera_centers(images)
+    C = np.array([centers[images[iid].name] for iid in order])
+    rays = C[:, None, :] - X[None, :, :]
+    rays /= np.linalg.norm(rays, axis=-1, keepdims=True)
     out = {}
     for ref, spec in cfg:
         if ref not in by_name:
             continue
+        a = row[by_name[ref]]
         others = [n for n in sorted(by_name) if n != ref]
         if isinstance(spec, list):
             out[ref] = [s for s in spec if s in by_name and s != ref]
@@ -103,9 +103,13 @@ def select_sources(mvs_dir, min_triangulation_angle=5.0):
         else:
             cand = []
             for n in others:
-                angs = shared.get((ref, n), [])
-                if angs and np.median(angs) >= min_triangulation_angle:
-                    cand.append((-len(angs), n))
+                b = row[by_name[n]]
+                shared = vis[a] & vis[b]
+                if not shared.any():
+                    continue
+                c = np.clip(np.sum(rays[a, shared] * rays[b, shared], axis=1), -1.0, 1.0)
+                if np.median(np.degrees(np.arccos(c))) >= min_triangulation_angle:
+                    cand.append((-int(shared.sum()), n))
             cand.sort()
             out[ref] = [n for _, n in cand[:spec[1]]]
     return out

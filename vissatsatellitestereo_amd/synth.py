@@ -82,6 +82,48 @@ def fbm(x, y, seed, wavelength, octaves, persistence=0.5):
     return out
 
 
+def fbm_grid(x0, y_top, res, n, seed, wavelength, octaves, persistence=0.5, device="cpu"):
+    """fbm on the n x n raster with column 0 at easting x0 and row 0 at
+    northing y_top (cell size res), computed with torch on `device`; the same
+    values as fbm() on that raster up to FP64 rounding."""
+    import torch
+    dev = torch.device(device)
+    xs = x0 + res * torch.arange(n, dtype=torch.float64, device=dev)
+    ys = y_top - res * torch.arange(n, dtype=torch.float64, device=dev)
+    Y, X = torch.meshgrid(ys, xs, indexing="ij")
+    g2 = torch.tensor(_GRAD2, dtype=torch.float64, device=dev)
+    out = torch.zeros((n, n), dtype=torch.float64, device=dev)
+    amp = 1.0
+    lam = float(wavelength)
+    for o in range(octaves):
+        perm = torch.from_numpy(_perm_table(seed * 7919 + o)).to(dev)
+        x = X / lam + 0.37 * o
+        y = Y / lam + 0.61 * o
+        xi = torch.floor(x)
+        yi = torch.floor(y)
+        xf = x - xi
+        yf = y - yi
+        xi = xi.long() & 255
+        yi = yi.long() & 255
+        u = xf * xf * xf * (xf * (xf * 6.0 - 15.0) + 10.0)
+        v = yf * yf * yf * (yf * (yf * 6.0 - 15.0) + 10.0)
+
+        def gr(ix, iy, dx, dy):
+            h = perm[perm[ix] + iy] & 7
+            return g2[h, 0] * dx + g2[h, 1] * dy
+
+        n00 = gr(xi, yi, xf, yf)
+        n10 = gr(xi + 1, yi, xf - 1.0, yf)
+        n01 = gr(xi, yi + 1, xf, yf - 1.0)
+        n11 = gr(xi + 1, yi + 1, xf - 1.0, yf - 1.0)
+        nx0 = n00 + u * (n10 - n00)
+        nx1 = n01 + u * (n11 - n01)
+        out += amp * (nx0 + v * (nx1 - nx0))
+        amp *= persistence
+        lam *= 0.5
+    return out.cpu().numpy()
+
+
 # ----------------------------------------------------------------------------
 # Scene description
 # ----------------------------------------------------------------------------
@@ -199,7 +241,7 @@ def make_camera(name, width, height, off_nadir_deg, azimuth_deg, focal,
 class Scene:
     """Rasterised heightfield + albedo, and the cameras that view it."""
 
-    def __init__(self, spec: SceneSpec):
+    def __init__(self, spec: SceneSpec, device="cpu"):
         self.spec = spec
         rng_cam = np.random.default_rng(spec.seed_cameras)
         sizes = spec.sizes or [(spec.width, spec.height)] * spec.n_views
@@ -233,23 +275,27 @@ class Scene:
         xs = -self.half + g * np.arange(n)
         self.grid_x0 = -self.half
         self.grid_n = n
-        X, Y = np.meshgrid(xs, xs[::-1], indexing="xy")   # row 0 = north
-        self._X, self._Y = X, Y
-        terr = fbm(X, Y, spec.seed_terrain, wavelength=max(120.0, L * 0.8), octaves=5, persistence=0.5)
+        terr = fbm_grid(-self.half, -self.half + g * (n - 1), g, n, spec.seed_terrain,
+                        wavelength=max(120.0, L * 0.8), octaves=5, persistence=0.5, device=device)
         terr *= spec.terrain_amp / max(1e-9, np.abs(terr).max())
         self.terrain = terr
         self.boxes = self._make_boxes(L)
         Hf = terr.copy()
-        for (cxb, cyb, w, l, ang, roof) in self.boxes:
-            m = self._in_box(X, Y, cxb, cyb, w, l, ang)
-            Hf[m] = np.maximum(Hf[m], roof)
+        y_top = -self.half + g * (n - 1)
+        for box in self.boxes:
+            r0, r1, c0, c1 = self._box_window(box, xs[0], y_top, g, n, n)
+            if r1 <= r0 or c1 <= c0:
+                continue
+            XX, YY = np.meshgrid(xs[c0:c1], y_top - g * np.arange(r0, r1), indexing="xy")
+            m = self._in_box(XX, YY, *box[:5])
+            blk = Hf[r0:r1, c0:c1]
+            blk[m] = np.maximum(blk[m], box[5])
         self.heightfield = Hf
         # Albedo texture over a wider domain (walls sample it at x + 0.37 z, y + 0.61 z).
         self.alb_half = self.half + 45.0
         na = int(round(2 * self.alb_half / g)) + 1
-        xa = -self.alb_half + g * np.arange(na)
-        XA, YA = np.meshgrid(xa, xa[::-1], indexing="xy")
-        a = fbm(XA, YA, spec.seed_albedo, wavelength=24.0, octaves=6, persistence=0.65)
+        a = fbm_grid(-self.alb_half, -self.alb_half + g * (na - 1), g, na, spec.seed_albedo,
+                     wavelength=24.0, octaves=6, persistence=0.65, device=device)
         a = a / max(1e-9, np.abs(a).max())
         self.albedo = 0.5 + 0.4 * a
         self.alb_n = na
@@ -278,6 +324,18 @@ class Scene:
         return boxes
 
     @staticmethod
+    def _box_window(box, x0, y_top, res, nrows, ncols):
+        """Row/column range of a raster (row 0 at northing y_top, column 0 at
+        easting x0, cell size res) that can contain the rotated box."""
+        cxb, cyb, w, l = box[:4]
+        rad = 0.5 * math.hypot(w, l) + res
+        c0 = max(0, int(math.floor((cxb - rad - x0) / res)))
+        c1 = min(ncols, int(math.ceil((cxb + rad - x0) / res)) + 1)
+        r0 = max(0, int(math.floor((y_top - (cyb + rad)) / res)))
+        r1 = min(nrows, int(math.ceil((y_top - (cyb - rad)) / res)) + 1)
+        return r0, r1, c0, c1
+
+    @staticmethod
     def _in_box(X, Y, cxb, cyb, w, l, ang):
         ca, sa = math.cos(ang), math.sin(ang)
         dx = X - cxb
@@ -296,7 +354,11 @@ class Scene:
         """Analytic surface height (terrain raster + exact box footprints)."""
         h = self.terrain_at(x, y)
         for (cxb, cyb, w, l, ang, roof) in self.boxes:
-            m = self._in_box(x, y, cxb, cyb, w, l, ang)
+            rad = 0.5 * math.hypot(w, l)
+            near = (np.abs(x - cxb) <= rad) & (np.abs(y - cyb) <= rad)
+            if not near.any():
+                continue
+            m = near & self._in_box(x, y, cxb, cyb, w, l, ang)
             h = np.where(m, np.maximum(h, roof), h)
         return h
 
@@ -317,24 +379,38 @@ class Scene:
         Cell (row, col) covers northing (ul_n - (row+1) res, ul_n - row res]
         and easting [ul_e + col res, ul_e + (col+1) res), the cell that
         ``floor((ul_n - N)/res)``, ``floor((E - ul_e)/res)`` assigns
-        (``lib/proj_to_grid.py:42-43``)."""
+        (``lib/proj_to_grid.py:42-43``).  Terrain and each box are sampled at
+        sub x sub points per cell; boxes only within their bounding windows."""
         aoi = self.aoi()
         e_size = int(aoi["width"] / resolution) + 1
         n_size = int(aoi["height"] / resolution) + 1
         offs = (np.arange(sub) + 0.5) / sub
         out = np.full((n_size, e_size), -np.inf)
         cols = np.arange(e_size)
+        ul_e, ul_n = aoi["ul_easting"], aoi["ul_northing"]
         for i in range(sub):
             for j in range(sub):
-                E = aoi["ul_easting"] + (cols + offs[j]) * resolution
+                E = ul_e + (cols + offs[j]) * resolution
                 for r0 in range(0, n_size, 512):
                     rows = np.arange(r0, min(n_size, r0 + 512))
-                    N = aoi["ul_northing"] - (rows + offs[i]) * resolution
+                    N = ul_n - (rows + offs[i]) * resolution
                     EE, NN = np.meshgrid(E, N, indexing="xy")
-                    h = self.true_height(EE, NN)
-                    out[rows] = np.maximum(out[rows], h)
+                    out[rows] = np.maximum(out[rows], self.terrain_at(EE, NN))
+        for box in self.boxes:
+            r0, r1, c0, c1 = self._box_window(box, ul_e, ul_n, resolution, n_size, e_size)
+            if r1 <= r0 or c1 <= c0:
+                continue
+            rr = np.arange(r0, r1)
+            cc = np.arange(c0, c1)
+            hit = np.zeros((r1 - r0, c1 - c0), dtype=bool)
+            for i in range(sub):
+                for j in range(sub):
+                    EE, NN = np.meshgrid(ul_e + (cc + offs[j]) * resolution,
+                                         ul_n - (rr + offs[i]) * resolution, indexing="xy")
+                    hit |= self._in_box(EE, NN, *box[:5])
+            blk = out[r0:r1, c0:c1]
+            blk[hit] = np.maximum(blk[hit], box[5])
         return out
-
 
 def _bilerp_np(img, row, col):
     h, w = img.shape
@@ -543,7 +619,7 @@ def build_site(spec: SceneSpec, work_dir: str, device="cpu", save_gt_views=True,
       work_dir/synthetic_gt/{gt_dsm.npy, <name>.hit_z.npy, <name>.zcam.npy}
     """
     from PIL import Image
-    scene = Scene(spec)
+    scene = Scene(spec, device=device)
     scene.cameras = cameras_with_exact_pose(scene.cameras)
     img_dir = os.path.join(work_dir, "colmap", "sfm_perspective", "images")
     sparse_dir = os.path.join(work_dir, "colmap", "sfm_perspective", "tri_ba")
@@ -582,6 +658,13 @@ def spec_c2(n_views=10, size=1024) -> SceneSpec:
     return SceneSpec(width=size, height=size, n_views=n_views, off_nadir=(5.0, 30.0),
                      azimuths=tuple(45.0 * k for k in range(8)), n_boxes=20,
                      n_points=3000)
+
+
+def spec_c5(n_views=40, size=4096) -> SceneSpec:
+    """Large site: 40 views of 4096x4096 over ~2 km^2 with 500 buildings."""
+    return SceneSpec(width=size, height=size, n_views=n_views, off_nadir=(5.0, 30.0),
+                     azimuths=tuple(45.0 * k for k in range(8)), n_boxes=500, n_points=6000,
+                     terrain_amp=15.0)
 
 
 def spec_ragged(n_views=32, lo=512, hi=2048, seed=5, n_boxes=40, seed_offset=0) -> SceneSpec:

@@ -106,6 +106,50 @@ typedef struct {
     int32_t e_size, n_size;
 } vsm_dsm_grid;
 
+/* Geodetic frame of the local ENU coordinates and UTM zone of the DSM grid.
+ * The ENU origin is the AOI centre at alt_min (coordinate_system.py:41-51:
+ * lat0 = (lat_min + lat_max)/2, lon0 = (lon_min + lon_max)/2, alt0 = alt_min);
+ * the UTM zone and hemisphere are aoi.json's (the reference takes them from
+ * the first point, lib/latlon_utm_converter.py:42-50, the same zone for an AOI
+ * inside one zone).  WGS84 throughout. */
+typedef struct {
+    double lat0_deg, lon0_deg, alt0;
+    int32_t zone_number;              /* 1..60 */
+    int32_t south;                    /* 0: 'N' (false northing 0), 1: 'S' (1e7 m) */
+} vsm_geo_params;
+
+/* FP64 constants of the ENU -> geodetic -> UTM chain, filled by
+ * vsm_geo_frame_init (30 doubles, no padding). */
+typedef struct {
+    double origin[3];                 /* ECEF of the ENU origin */
+    double rot[9];                    /* ECEF offset = rot * (e, n, u), row-major */
+    double cm_cos, cm_sin;            /* central meridian of the zone */
+    double a, e, e2, e2a, ep2b, one_minus_f;   /* WGS84: a, e, e^2, e^2 a, e'^2 b, 1 - f */
+    double k0A;                       /* 0.9996 * rectifying radius */
+    double alpha[6];                  /* Krueger series coefficients (order n^6) */
+    double false_easting, false_northing;
+    double max_q;                     /* domain of the series (see vsm_enu_to_utm) */
+} vsm_geo_frame;
+
+/* Host-only: derive the frame constants.  Returns VSM_EINVAL for a zone
+ * outside 1..60 or |lat0| > 84 degrees. */
+int vsm_geo_frame_init(const vsm_geo_params *params, vsm_geo_frame *out);
+
+/* Select the frame of the DSM grid for vsm_fuse_dsm: NULL (the default) keeps
+ * the local ENU grid of the back-projected points; a frame makes every
+ * back-projected point (E_enu, N_enu, U) go through local_to_global +
+ * latlon_to_eastnorh (aggregate_2p5d_util.py:95-98) first, so the grid is the
+ * UTM grid of aoi.json and the cell value the WGS84 altitude. */
+int vsm_set_geo_frame(vsm_ctx *ctx, const vsm_geo_frame *frame);
+
+/* Stand-alone conversion (coordinate_system.py:41-51 followed by
+ * lib/latlon_utm_converter.py:39-52) of n local ENU points (n x 3 FP64) to
+ * n x 3 (easting, northing, altitude) on the device, with the frame set by
+ * vsm_set_geo_frame (VSM_EINVAL if none).  A point more than about 14 degrees
+ * of longitude from the central meridian is outside the series' domain and
+ * comes back as NaN. */
+int vsm_enu_to_utm(vsm_ctx *ctx, const double *enu, int64_t n, double *out);
+
 /* Kernel timing and work counters accumulated since the last reset. */
 typedef struct {
     double halfstep_ms;               /* device time of half-step kernels */
@@ -191,6 +235,8 @@ int vsm_debug_filter(vsm_ctx *ctx, const vsm_view_req *req, int32_t pass, float 
  * (produce_dsm.py:58); across views the <= (min_measurements-1) rejection,
  * MAD outlier removal, mean and 3x3 median (aggregate_2p5d.py:66-81).
  * depth_kind: 0 photometric, 1 geometric (aggregate_2p5d.py:48).
+ * With a geo frame set (vsm_set_geo_frame) each back-projected ENU point is
+ * converted to UTM (E, N) and WGS84 altitude before it is rasterised.
  * inv_proj_mats: optional n*16 FP64 row-major inverse projections as read
  * from inv_proj_mats.txt (aggregate_2p5d_util.py:55-61); NULL uses the
  * context's own inverse of each P4.

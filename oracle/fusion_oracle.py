@@ -28,6 +28,12 @@ Follows, function by function:
                               in FP64 (the reference's float32 nanmean order
                               is numpy-internal)
 * ``grid_from_aoi``           produce_dsm.py:51-55
+* ``view_dsm(..., frame)``    with a geodetic frame (30 constants of
+                              vsm_geo_frame) the back-projected ENU points go
+                              through local_to_global + latlon_to_eastnorh
+                              (aggregate_2p5d_util.py:95-98, geo_oracle.c)
+                              before the grid, which is then aoi.json's UTM
+                              grid; points outside the series' domain drop out
 """
 import json
 import math
@@ -124,8 +130,13 @@ def produce_dsm_from_points(points, grid):
     return median_blur3(dsm.astype(np.float32))
 
 
-def view_dsm(depth, inv_proj, grid):
-    return produce_dsm_from_points(backproject(depth, inv_proj), grid)
+def view_dsm(depth, inv_proj, grid, frame=None):
+    pts = backproject(depth, inv_proj)
+    if frame is not None:
+        import geo_oracle
+        pts = geo_oracle.enu_to_utm(frame, pts)
+        pts = pts[~np.isnan(pts[:, 0])]
+    return produce_dsm_from_points(pts, grid)
 
 
 def fuse(stack, min_measurements=3):
@@ -161,19 +172,33 @@ def read_inv_proj_mats(path):
     return out
 
 
-def run_fuse(work_dir, depth_type="geometric", min_measurements=3, resolution=0.5):
-    """Whole-site fusion from the workspace files; returns (fused, per_view)."""
+def geo_frame_from_aoi(aoi):
+    """Frame constants of a UTM aoi.json (None for a local-ENU one)."""
+    import geo_oracle
+    if int(aoi.get("zone_number", 0)) == 0 or str(aoi.get("frame", "")).startswith("local"):
+        return None
+    return geo_oracle.frame_constants((aoi["lat_min"] + aoi["lat_max"]) / 2.0,
+                                      (aoi["lon_min"] + aoi["lon_max"]) / 2.0, float(aoi["alt_min"]),
+                                      int(aoi["zone_number"]), 0 if aoi["hemisphere"] == "N" else 1)
+
+
+def run_fuse(work_dir, depth_type="geometric", min_measurements=3, resolution=0.5, frame="aoi"):
+    """Whole-site fusion from the workspace files; returns (fused, per_view).
+    frame: "aoi" derives the geodetic frame from aoi.json, else the 30
+    constants or None (local ENU grid)."""
     mvs = os.path.join(work_dir, "colmap", "mvs")
     inv = read_inv_proj_mats(os.path.join(mvs, "inv_proj_mats.txt"))
     with open(os.path.join(work_dir, "aoi.json")) as fp:
         aoi = json.load(fp)
     grid = grid_from_aoi(aoi, resolution, resolution)
+    if isinstance(frame, str):
+        frame = geo_frame_from_aoi(aoi)
     ddir = os.path.join(mvs, "stereo", "depth_maps")
     items = sorted(i for i in os.listdir(ddir) if depth_type in i)
     per_view = []
     for item in items:
         name = item[:item.rfind(f".{depth_type}.bin")]
-        per_view.append(view_dsm(read_array(os.path.join(ddir, item)), inv[name], grid))
+        per_view.append(view_dsm(read_array(os.path.join(ddir, item)), inv[name], grid, frame))
     stack = np.stack(per_view)
     return fuse(stack, min_measurements), stack
 

@@ -16,13 +16,14 @@ def tiny_spec():
                            azimuths=(0.0, 90.0, 180.0, 270.0), n_boxes=6, n_points=800)
 
 
-def build(name, spec, root, max_src=20):
+def build(name, spec, root, max_src=20, geo=None):
+    """geo=(lat, lon) anchors the site in UTM (aoi.json with a real zone)."""
     key = (name, str(root))
     if key in _CACHE:
         return _CACHE[key]
     wd = os.path.join(str(root), name)
     if not os.path.exists(os.path.join(wd, "colmap", "mvs", "depth_ranges.txt")):
-        synth.build_site(spec, wd)
+        synth.build_site(spec, wd, geo=geo)
         workspace.prepare_dense_workspace(wd, max_src=max_src)
     mvs = os.path.join(wd, "colmap", "mvs")
     P4 = rd.projection_matrices(mvs)

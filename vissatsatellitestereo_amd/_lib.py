@@ -64,6 +64,26 @@ class DsmGrid(ctypes.Structure):
                 ("n_res", ctypes.c_double), ("e_size", ctypes.c_int32), ("n_size", ctypes.c_int32)]
 
 
+class GeoParams(ctypes.Structure):
+    _fields_ = [("lat0_deg", ctypes.c_double), ("lon0_deg", ctypes.c_double), ("alt0", ctypes.c_double),
+                ("zone_number", ctypes.c_int32), ("south", ctypes.c_int32)]
+
+
+class GeoFrame(ctypes.Structure):
+    _fields_ = [("origin", ctypes.c_double * 3), ("rot", ctypes.c_double * 9),
+                ("cm_cos", ctypes.c_double), ("cm_sin", ctypes.c_double),
+                ("a", ctypes.c_double), ("e", ctypes.c_double), ("e2", ctypes.c_double),
+                ("e2a", ctypes.c_double), ("ep2b", ctypes.c_double), ("one_minus_f", ctypes.c_double),
+                ("k0A", ctypes.c_double), ("alpha", ctypes.c_double * 6),
+                ("false_easting", ctypes.c_double), ("false_northing", ctypes.c_double),
+                ("max_q", ctypes.c_double)]
+
+    def as_array(self):
+        """The 30 constants in declaration order (FP64)."""
+        import numpy as np
+        return np.frombuffer(bytes(self), dtype=np.float64).copy()
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("halfstep_ms", ctypes.c_double), ("halfstep_launches", ctypes.c_int64),
                 ("ncc_evals", ctypes.c_int64), ("geo_evals", ctypes.c_int64),
@@ -108,11 +128,24 @@ SIGNATURES = [
                                     P(_c.c_double)]),
     ("vsm_median_blur3", _c.c_int, [_c.c_void_p, P(_c.c_float), _c.c_int32, _c.c_int32,
                                     P(_c.c_float)]),
+    ("vsm_geo_frame_init", _c.c_int, [P(GeoParams), P(GeoFrame)]),
+    ("vsm_set_geo_frame", _c.c_int, [_c.c_void_p, P(GeoFrame)]),
+    ("vsm_enu_to_utm", _c.c_int, [_c.c_void_p, P(_c.c_double), _c.c_int64, P(_c.c_double)]),
     ("vsm_get_stats", _c.c_int, [_c.c_void_p, P(Stats)]),
     ("vsm_reset_stats", _c.c_int, [_c.c_void_p]),
     ("vsm_set_timing", _c.c_int, [_c.c_void_p, _c.c_int32]),
     ("vsm_synchronize", _c.c_int, [_c.c_void_p]),
 ]
+
+def geo_frame(lat0_deg, lon0_deg, alt0, zone_number, south):
+    """vsm_geo_frame_init: the FP64 constants of the ENU -> UTM chain."""
+    p = GeoParams(float(lat0_deg), float(lon0_deg), float(alt0), int(zone_number), int(south))
+    f = GeoFrame()
+    rc = load().vsm_geo_frame_init(ctypes.byref(p), ctypes.byref(f))
+    if rc != 0:
+        raise ValueError(f"vsm_geo_frame_init rejected {lat0_deg, lon0_deg, alt0, zone_number, south}")
+    return f
+
 
 _LIB = None
 _LOCK = threading.Lock()

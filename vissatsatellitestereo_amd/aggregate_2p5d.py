@@ -4,8 +4,10 @@
 DSMs of the geometric depth maps, then per cell the <= 2-measurement
 rejection (aggregate_2p5d.py:69-71), MAD outlier removal and mean
 (aggregate_2p5d.py:74-78) and a 3x3 median (aggregate_2p5d.py:81), written
-to ``mvs_results/aggregate_2p5d/aggregate_2p5d_dsm.tif`` (float32 TIFF, NaN
-= empty; plus ``.npy``).  The whole chain runs on the device in one call.
+to ``mvs_results/aggregate_2p5d/aggregate_2p5d_dsm.tif`` (GeoTIFF, nodata
+-10000 as produce_dsm_from_height writes it, produce_dsm.py:71-82; plus an
+``.npy`` with NaN = empty).  The whole chain runs on the device in one call;
+for a UTM aoi.json the grid is UTM (see aggregate_2p5d_util).
 
 Keyword extensions: ``depth_type`` (the reference hard-codes 'geometric',
 aggregate_2p5d.py:48) and ``min_measurements`` (3 reproduces the reference's
@@ -29,7 +31,8 @@ def run_fuse(work_dir, max_processes=-1, depth_type="geometric", min_measurement
         logging.info("dsm %s empty ratio: %s", name, float(np.isnan(dsm).mean()))
     out_dir = os.path.join(work_dir, "mvs_results", "aggregate_2p5d")
     os.makedirs(out_dir, exist_ok=True)
-    runtime.write_tif(fused, os.path.join(out_dir, "aggregate_2p5d_dsm.tif"))
+    runtime.write_dsm_tif_aoi(fused, os.path.join(out_dir, "aggregate_2p5d_dsm.tif"),
+                              runtime.read_aoi(work_dir))
     np.save(os.path.join(out_dir, "aggregate_2p5d_dsm.npy"), fused)
     logging.info("After aggregation, empty ratio: %s", float(np.isnan(fused).mean()))
     return fused

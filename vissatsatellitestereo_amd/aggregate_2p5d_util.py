@@ -6,17 +6,21 @@ into a per-view DSM written to ``out_dir/dsm_tif/<name>.tif``: back-projection
 through inv(P4) from inv_proj_mats.txt (aggregate_2p5d_util.py:55-93),
 per-cell nanmax with hole fill (lib/proj_to_grid.py:41-81) and 3x3 median
 (produce_dsm.py:58), all in one device pass per view instead of a process
-pool.  The DSM grid is the aoi.json grid in the frame of the back-projected
-points (local ENU); the ENU -> geodetic -> UTM conversion of
-aggregate_2p5d_util.py:96-98 is not part of this path.  max_processes is
-accepted for signature compatibility; the GPU needs no worker pool.
+pool.  For a UTM aoi.json (the reference's, stereo_pipeline.py:185-226) every
+back-projected ENU point goes through local_to_global + latlon_to_eastnorh
+(aggregate_2p5d_util.py:95-98) on the device before it is rasterised, so the
+DSM is aoi.json's UTM grid of WGS84 altitudes; a local-ENU aoi.json
+(zone_number 0, the synthetic sites) keeps the grid in the ENU frame.  The
+per-view DSMs are written as GeoTIFFs with nodata -10000 (produce_dsm.py:59-61,
+lib/dsm_util.py:122-159).  max_processes is accepted for signature
+compatibility; the GPU needs no worker pool.
 """
 import os
 import shutil
 
 import numpy as np
 
-from . import runtime
+from . import geodesy, runtime
 
 
 def _items(mvs_dir, depth_type):
@@ -49,10 +53,15 @@ def per_view_dsms(work_dir, depth_type="geometric", min_measurements=3, resoluti
         depth = ws.read_depth(name, depth_type)
         ws.register(ctx, name, depth.shape)
         ctx.set_depth_map(ws.ids[name], depth, kind=kind)
-    grid = runtime.dsm_grid(runtime.read_aoi(work_dir), resolution, resolution)
+    aoi = runtime.read_aoi(work_dir)
+    grid = runtime.dsm_grid(aoi, resolution, resolution)
     mats = np.stack([inv[n] for n in names])
-    fused, pv = ctx.fuse_dsm(grid, [ws.ids[n] for n in names], depth_kind=kind,
-                             min_measurements=min_measurements, inv_proj_mats=mats, per_view=True)
+    ctx.set_geo_frame(geodesy.geo_params_from_aoi(aoi))
+    try:
+        fused, pv = ctx.fuse_dsm(grid, [ws.ids[n] for n in names], depth_kind=kind,
+                                 min_measurements=min_measurements, inv_proj_mats=mats, per_view=True)
+    finally:
+        ctx.set_geo_frame(None)
     return names, fused, pv
 
 
@@ -62,6 +71,7 @@ def convert_depth_maps(work_dir, out_dir, depth_type, max_processes=-1, min_meas
     for sub in (out_dir, os.path.join(out_dir, "dsm_tif")):
         os.makedirs(sub, exist_ok=True)
     names, fused, pv = per_view_dsms(work_dir, depth_type, min_measurements)
+    aoi = runtime.read_aoi(work_dir)
     for name, dsm in zip(names, pv):
-        runtime.write_tif(dsm, os.path.join(out_dir, "dsm_tif", name[:-4] + ".tif"))
+        runtime.write_dsm_tif_aoi(dsm, os.path.join(out_dir, "dsm_tif", name[:-4] + ".tif"), aoi)
     return names, fused, pv

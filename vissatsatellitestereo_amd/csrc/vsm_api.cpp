@@ -82,6 +82,8 @@ struct vsm_ctx {
     int64_t halfstep_launches = 0;
     double fuse_ms = 0.0;
     int64_t fuse_launches = 0, fuse_pixels = 0;
+    bool has_geo = false;                  // DSM grid in UTM (vsm_set_geo_frame)
+    vsm_geo_frame geo{};
     int64_t taps_per_eval = 0;
     int64_t acc_ncc = 0, acc_geo = 0, acc_taps = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -998,7 +1000,8 @@ int vsm_fuse_dsm(vsm_ctx *c, const vsm_dsm_grid *grid, const int32_t *ids, int32
     for (int v = 0; v < n; ++v) {
         const ImageRec &r = c->images[ids[v]];
         HIPCHK(c, launch_fill_u64n(cells, ncell, 0ull, c->stream));
-        HIPCHK(c, launch_rasterize_view(r.depth[depth_kind], r.W, r.H, mats + 16 * v, g, cells, c->stream));
+        HIPCHK(c, launch_rasterize_view(r.depth[depth_kind], r.W, r.H, mats + 16 * v, g,
+                                        c->has_geo ? &c->geo : nullptr, cells, c->stream));
         HIPCHK(c, launch_decode_fill_f32(cells, g.e_size, g.n_size, tmp, c->stream));
         HIPCHK(c, launch_median3(tmp, g.e_size, g.n_size, stack + (size_t)v * ncell, c->stream));
         pixels += (int64_t)r.W * r.H;
@@ -1045,6 +1048,96 @@ int vsm_proj_to_grid(vsm_ctx *c, const double *points, int64_t n, const vsm_dsm_
     HIPCHK(c, hipMemcpyAsync(dsm_out, out, ncell * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     dfree(c, cells); dfree(c, out); dfree(c, pts);
+    return VSM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Geodetic frame (coordinate_system.py:41-51, lib/latlon_utm_converter.py:39-52)
+// ---------------------------------------------------------------------------
+int vsm_geo_frame_init(const vsm_geo_params *p, vsm_geo_frame *out)
+{
+    if (!p || !out) return VSM_EINVAL;
+    if (p->zone_number < 1 || p->zone_number > 60 || !(std::fabs(p->lat0_deg) <= 84.0) ||
+        !(std::fabs(p->lon0_deg) <= 180.0) || !std::isfinite(p->alt0))
+        return VSM_EINVAL;
+    const double deg = M_PI / 180.0;
+    const double a = 6378137.0, f = 1.0 / 298.257223563;
+    const double e2 = f * (2.0 - f), b = a * (1.0 - f), ep2 = e2 / (1.0 - e2);
+    const double lat0 = p->lat0_deg * deg, lon0 = p->lon0_deg * deg;
+    const double sf = std::sin(lat0), cf = std::cos(lat0), sl = std::sin(lon0), cl = std::cos(lon0);
+    // geodetic -> ECEF of the origin
+    const double N0 = a / std::sqrt(1.0 - e2 * sf * sf);
+    vsm_geo_frame g{};
+    g.origin[0] = (N0 + p->alt0) * cf * cl;
+    g.origin[1] = (N0 + p->alt0) * cf * sl;
+    g.origin[2] = (N0 * (1.0 - e2) + p->alt0) * sf;
+    // columns: east, north, up unit vectors in ECEF
+    const double R[9] = {-sl, -sf * cl, cf * cl,
+                         cl,  -sf * sl, cf * sl,
+                         0.0, cf,       sf};
+    memcpy(g.rot, R, sizeof R);
+    const double lam0 = (6.0 * p->zone_number - 183.0) * deg;
+    g.cm_cos = std::cos(lam0);
+    g.cm_sin = std::sin(lam0);
+    g.a = a;
+    g.e2 = e2;
+    g.e = std::sqrt(e2);
+    g.e2a = e2 * a;
+    g.ep2b = ep2 * b;
+    g.one_minus_f = 1.0 - f;
+    // Krueger series to order n^6 (Karney 2011, eq. 35)
+    const double n = f / (2.0 - f), n2 = n * n, n3 = n2 * n, n4 = n3 * n, n5 = n4 * n, n6 = n5 * n;
+    const double A = a / (1.0 + n) * (1.0 + n2 / 4.0 + n4 / 64.0 + n6 / 256.0);
+    g.k0A = 0.9996 * A;
+    g.alpha[0] = n / 2.0 - 2.0 * n2 / 3.0 + 5.0 * n3 / 16.0 + 41.0 * n4 / 180.0 -
+                 127.0 * n5 / 288.0 + 7891.0 * n6 / 37800.0;
+    g.alpha[1] = 13.0 * n2 / 48.0 - 3.0 * n3 / 5.0 + 557.0 * n4 / 1440.0 + 281.0 * n5 / 630.0 -
+                 1983433.0 * n6 / 1935360.0;
+    g.alpha[2] = 61.0 * n3 / 240.0 - 103.0 * n4 / 140.0 + 15061.0 * n5 / 26880.0 +
+                 167603.0 * n6 / 181440.0;
+    g.alpha[3] = 49561.0 * n4 / 161280.0 - 179.0 * n5 / 168.0 + 6601661.0 * n6 / 7257600.0;
+    g.alpha[4] = 34729.0 * n5 / 80640.0 - 3418889.0 * n6 / 1995840.0;
+    g.alpha[5] = 212378941.0 * n6 / 319334400.0;
+    g.false_easting = 500000.0;
+    g.false_northing = p->south ? 10000000.0 : 0.0;
+    g.max_q = 0.25;
+    *out = g;
+    return VSM_OK;
+}
+
+int vsm_set_geo_frame(vsm_ctx *c, const vsm_geo_frame *frame)
+{
+    if (!c) return VSM_EINVAL;
+    if (!frame) {
+        c->has_geo = false;
+        return VSM_OK;
+    }
+    const double *v = reinterpret_cast<const double *>(frame);
+    for (size_t i = 0; i < sizeof(vsm_geo_frame) / sizeof(double); ++i)
+        if (!std::isfinite(v[i])) return fail(c, VSM_EINVAL, "non-finite geo frame constant %zu", i);
+    if (!(frame->a > 0.0) || !(frame->max_q > 0.0) || !(frame->max_q <= 0.25))
+        return fail(c, VSM_EINVAL, "geo frame: bad ellipsoid or max_q (0 < max_q <= 0.25)");
+    c->geo = *frame;
+    c->has_geo = true;
+    return VSM_OK;
+}
+
+int vsm_enu_to_utm(vsm_ctx *c, const double *enu, int64_t n, double *out)
+{
+    if (!c) return VSM_EINVAL;
+    if (!c->has_geo) return fail(c, VSM_EINVAL, "no geo frame set");
+    if (n < 0 || (n > 0 && (!enu || !out))) return fail(c, VSM_EINVAL, "bad points/output");
+    if (n == 0) return VSM_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    double *buf = nullptr;
+    int rc = dalloc(c, &buf, 6 * (size_t)n);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(buf, enu, 24 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_enu_to_utm(buf, n, c->geo, buf + 3 * (size_t)n, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out, buf + 3 * (size_t)n, 24 * (size_t)n, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(c, buf);
     return VSM_OK;
 }
 

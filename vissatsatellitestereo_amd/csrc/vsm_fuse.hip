@@ -18,6 +18,7 @@
 // Median rule (NaN-aware): NaN excluded; even count -> mean of the two middle
 // values; no values -> NaN.
 #include "vsm_device.h"
+#include "vsm_geo.h"
 #include "vsm_kernels.h"
 
 namespace vsm {
@@ -44,9 +45,12 @@ __device__ __forceinline__ void cell_of(const DsmGridDev &g, double E, double N,
 }
 
 // One thread per depth pixel; d <= 0 or non-finite -> no point
-// (aggregate_2p5d_util.py:76, 92-93).
+// (aggregate_2p5d_util.py:76, 92-93).  GEO: the ENU point goes through
+// local_to_global + latlon_to_eastnorh (aggregate_2p5d_util.py:95-98) and the
+// cell value is the altitude.
+template <bool GEO>
 __global__ void k_rasterize_view(const float *__restrict__ depth, int W, int H,
-                                 const double *__restrict__ Mi, DsmGridDev g,
+                                 const double *__restrict__ Mi, DsmGridDev g, vsm_geo_frame gf,
                                  unsigned long long *__restrict__ cell_max)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -58,8 +62,15 @@ __global__ void k_rasterize_view(const float *__restrict__ depth, int W, int H,
 #pragma unroll
     for (int i = 0; i < 4; ++i)
         X[i] = ((Mi[4 * i + 0] * col + Mi[4 * i + 1] * row) + Mi[4 * i + 2]) + Mi[4 * i + 3] * d;
-    const double E = X[0] / X[3], N = X[1] / X[3], h = X[2] / X[3];
+    double E = X[0] / X[3], N = X[1] / X[3], h = X[2] / X[3];
     if (!isfinite(E) || !isfinite(N) || !isfinite(h)) return;
+    if (GEO) {
+        double Eu, Nu, alt;
+        if (!geo::enu_to_utm(gf, E, N, h, Eu, Nu, alt)) return;
+        E = Eu;
+        N = Nu;
+        h = alt;
+    }
     int r, c;
     bool ok;
     cell_of(g, E, N, r, c, ok);
@@ -79,6 +90,19 @@ __global__ void k_rasterize_points(const double *__restrict__ pts, long long n, 
     cell_of(g, E, N, r, c, ok);
     if (!ok) return;
     atomicMax(&cell_max[(size_t)r * g.e_size + c], enc_f64(h));
+}
+
+__global__ void k_enu_to_utm(const double *__restrict__ enu, long long n, vsm_geo_frame gf,
+                             double *__restrict__ out)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double E, N, alt;
+    const bool ok = geo::enu_to_utm(gf, enu[3 * i], enu[3 * i + 1], enu[3 * i + 2], E, N, alt);
+    const double nan = __longlong_as_double(0x7FF8000000000000ll);
+    out[3 * i] = ok ? E : nan;
+    out[3 * i + 1] = ok ? N : nan;
+    out[3 * i + 2] = ok ? alt : nan;
 }
 
 // Small in-register sort (insertion) of up to 9 doubles/floats.
@@ -231,10 +255,23 @@ __global__ void k_fill_u64n(unsigned long long *p, long long n, unsigned long lo
 static inline unsigned int nblk(long long n, int b) { return (unsigned int)((n + b - 1) / b); }
 
 hipError_t launch_rasterize_view(const float *depth, int W, int H, const double *invP4_dev,
-                                 DsmGridDev g, unsigned long long *cell_max, hipStream_t st)
+                                 DsmGridDev g, const vsm_geo_frame *geo,
+                                 unsigned long long *cell_max, hipStream_t st)
 {
-    k_rasterize_view<<<nblk((long long)W * H, 256), 256, 0, st>>>(depth, W, H, invP4_dev, g,
-                                                                  cell_max);
+    if (geo)
+        k_rasterize_view<true><<<nblk((long long)W * H, 256), 256, 0, st>>>(depth, W, H, invP4_dev,
+                                                                            g, *geo, cell_max);
+    else
+        k_rasterize_view<false><<<nblk((long long)W * H, 256), 256, 0, st>>>(
+            depth, W, H, invP4_dev, g, vsm_geo_frame{}, cell_max);
+    return hipGetLastError();
+}
+
+hipError_t launch_enu_to_utm(const double *enu, long long n, const vsm_geo_frame &geo, double *out,
+                             hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    k_enu_to_utm<<<nblk(n, 256), 256, 0, st>>>(enu, n, geo, out);
     return hipGetLastError();
 }
 

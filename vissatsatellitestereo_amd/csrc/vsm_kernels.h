@@ -1,6 +1,7 @@
 // vsm_kernels.h -- host-side launchers of the HIP kernels (internal).
 #pragma once
 #include "vsm_device.h"
+#include "../../include/vissat_mvs.h"
 
 namespace vsm {
 
@@ -28,8 +29,12 @@ struct DsmGridDev {
     double ul_e, ul_n, e_res, n_res;
     int e_size, n_size;
 };
+// geo: NULL = grid in the ENU frame of the points; else the UTM grid.
 hipError_t launch_rasterize_view(const float *depth, int W, int H, const double *invP4_dev,
-                                 DsmGridDev g, unsigned long long *cell_max, hipStream_t st);
+                                 DsmGridDev g, const vsm_geo_frame *geo,
+                                 unsigned long long *cell_max, hipStream_t st);
+hipError_t launch_enu_to_utm(const double *enu, long long n, const vsm_geo_frame &geo, double *out,
+                             hipStream_t st);
 hipError_t launch_rasterize_points(const double *pts, long long n, DsmGridDev g,
                                    unsigned long long *cell_max, hipStream_t st);
 hipError_t launch_decode_fill_f32(const unsigned long long *cell_max, int e_size, int n_size,

@@ -179,9 +179,9 @@ constexpr float kDeltaMax = 1e-3f;
 // reciprocal (fallback when some lane's window exceeds kDeltaMax).
 template <int G, bool EXACT, bool CLAMP>
 __device__ __forceinline__ void window_pass(const PMConst &c, const SrcLds &S, const Proj (&P)[G],
-                                            const float (&iz0)[G], const float *s_ref, int lbase,
-                                            const RefStats &R, float (&ss)[G], float (&sss)[G],
-                                            float (&srs)[G])
+                                            const float (&iz0)[G], const float (&sc)[G],
+                                            const float *s_ref, int lbase, const RefStats &R,
+                                            float (&ss)[G], float (&sss)[G], float (&srs)[G])
 {
     const int rad = c.radius, st = c.step, nt = c.ntaps;
     const float frad = (float)(-rad);
@@ -221,7 +221,7 @@ __device__ __forceinline__ void window_pass(const PMConst &c, const SrcLds &S, c
                 // windows proven inside the image skip it
                 const float px = CLAMP ? __builtin_amdgcn_fmed3f(qx * iz, S.lox, S.hix) : qx * iz;
                 const float py = CLAMP ? __builtin_amdgcn_fmed3f(qy * iz, S.loy, S.hiy) : qy * iz;
-                const float sv = sample_quad(S, px, py) - R.rc;
+                const float sv = sample_quad(S, px, py) - sc[k];
                 const float ws = w * sv;
                 ss[k] += ws;
                 sss[k] = fmaf(ws, sv, sss[k]);
@@ -231,16 +231,18 @@ __device__ __forceinline__ void window_pass(const PMConst &c, const SrcLds &S, c
     }
 }
 
-// NCC cost of G plane hypotheses against source S.  Source samples are
-// centred on the reference centre value rc before accumulation (NCC is
-// invariant to the offset; it only keeps the FP32 sums well conditioned).
+// NCC cost of G plane hypotheses against source S.  Each hypothesis' source
+// samples are centred on its own window-centre sample sc, the reference
+// samples on the reference centre value (NCC is invariant to the offsets;
+// they keep the FP32 moment sums well conditioned when the two views differ
+// in brightness).
 template <int G>
 __device__ __forceinline__ void ncc_cost_group(const PMConst &c, const SrcLds &S, const Proj (&P)[G],
                                                const float *s_ref, int lbase, const RefStats &R,
                                                float (&cost)[G])
 {
     bool valid[G];
-    float iz0[G], ss[G], sss[G], srs[G];
+    float iz0[G], sc[G], ss[G], sss[G], srs[G];
     bool far = false, edge = false;
     const float fr = (float)c.radius;
 #pragma unroll
@@ -249,6 +251,7 @@ __device__ __forceinline__ void ncc_cost_group(const PMConst &c, const SrcLds &S
         iz0[k] = valid[k] ? __builtin_amdgcn_rcpf(P[k].q0z) : 1.f;
         const float cx = P[k].q0x * iz0[k], cy = P[k].q0y * iz0[k];
         valid[k] = valid[k] && cx >= S.lox && cx <= S.hix && cy >= S.loy && cy <= S.hiy;
+        sc[k] = valid[k] ? sample_quad(S, cx, cy) : 0.f;
         const float delta = fr * (fabsf(P[k].huz) + fabsf(P[k].hvz)) * fabsf(iz0[k]);
         far = far || !(delta <= kDeltaMax);
         // conservative half-extent of the projected window (+1 px guard)
@@ -259,11 +262,11 @@ __device__ __forceinline__ void ncc_cost_group(const PMConst &c, const SrcLds &S
         edge = edge || !(cx - mx >= S.lox && cx + mx <= S.hix && cy - my >= S.loy && cy + my <= S.hiy);
     }
     if (__ballot(far) != 0ull)
-        window_pass<G, true, true>(c, S, P, iz0, s_ref, lbase, R, ss, sss, srs);
+        window_pass<G, true, true>(c, S, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
     else if (__ballot(edge) != 0ull)
-        window_pass<G, false, true>(c, S, P, iz0, s_ref, lbase, R, ss, sss, srs);
+        window_pass<G, false, true>(c, S, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
     else
-        window_pass<G, false, false>(c, S, P, iz0, s_ref, lbase, R, ss, sss, srs);
+        window_pass<G, false, false>(c, S, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
 #pragma unroll
     for (int k = 0; k < G; ++k) {
         const float ms = ss[k] * R.inv_sw;

@@ -191,6 +191,26 @@ class MvsContext:
             len(pts), ctypes.byref(g), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
 
+    def set_geo_frame(self, geo):
+        """geo: None (local ENU DSM grid), a GeoFrame, or the tuple
+        (lat0_deg, lon0_deg, alt0, zone_number, south)."""
+        if geo is None:
+            self._check(self.lib.vsm_set_geo_frame(self.h, None))
+            return None
+        f = geo if isinstance(geo, _lib.GeoFrame) else _lib.geo_frame(*geo)
+        self._check(self.lib.vsm_set_geo_frame(self.h, ctypes.byref(f)))
+        return f
+
+    def enu_to_utm(self, enu):
+        """n x 3 local ENU -> n x 3 (easting, northing, altitude) on the device."""
+        pts = np.ascontiguousarray(enu, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros_like(pts)
+        if len(pts):
+            self._check(self.lib.vsm_enu_to_utm(
+                self.h, pts.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(pts),
+                out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
     def median_blur3(self, img):
         a = np.ascontiguousarray(img, dtype=np.float32)
         out = np.zeros_like(a)

@@ -129,10 +129,16 @@ def dsm_grid(aoi, e_resolution=0.5, n_resolution=0.5):
     return (aoi["ul_easting"], aoi["ul_northing"], e_resolution, n_resolution, e_size, n_size)
 
 
-def write_tif(arr, path):
-    """Float32 TIFF (no geo tags; the GeoTIFF writer is not part of this path)."""
-    Image.fromarray(np.ascontiguousarray(arr, dtype=np.float32), mode="F").save(path)
+def write_dsm_tif_aoi(arr, path, aoi, resolution=0.5, nodata_val=-10000):
+    """GeoTIFF of a DSM on the aoi.json grid, nodata -10000 as at
+    produce_dsm.py:59-61 and produce_dsm.py:80-82 (lib/dsm_util.py:122-159)."""
+    from .lib.dsm_util import write_dsm_tif
+    write_dsm_tif(np.asarray(arr, dtype=np.float32), path,
+                  (aoi["ul_easting"], aoi["ul_northing"], resolution, resolution),
+                  (int(aoi.get("zone_number", 0)), aoi.get("hemisphere", "N")), nodata_val=nodata_val)
 
 
 def read_tif(path):
-    return np.array(Image.open(path), dtype=np.float32)
+    """DSM raster of a GeoTIFF written by write_dsm_tif_aoi (nodata -> NaN)."""
+    from .lib.dsm_util import read_dsm_tif
+    return read_dsm_tif(path)[0]

@@ -373,6 +373,45 @@ class Scene:
                 "alt_min": float(math.floor(self.z_min)), "alt_max": float(math.ceil(self.z_max)),
                 "frame": "local ENU (east, north, up) metres"}
 
+    def geo_aoi(self, lat_c=30.3, lon_c=-81.2) -> dict:
+        """aoi.json in UTM (stereo_pipeline.py:185-226) for the site placed
+        with its ENU origin near (lat_c, lon_c): the UTM box of side
+        aoi_side is centred on the UTM coordinates of (lat_c, lon_c), the
+        ENU origin is the centre of the box's lat/lon bounds
+        (coordinate_system.py:41-51) at altitude alt_min = 0, so the
+        scene's z is the altitude up to the Earth's curvature."""
+        from . import geodesy
+        L = self.aoi_side
+        zone = geodesy.utm_zone(lat_c, lon_c)
+        hemi = "N" if lat_c >= 0 else "S"
+        E, N = geodesy.latlon_to_utm(lat_c, lon_c, zone, hemi == "N")
+        return geodesy.aoi_dict(zone, hemi, float(np.round(E - L / 2, 1)), float(np.round(N + L / 2, 1)),
+                                L, L, 0.0, float(math.ceil(self.z_max)))
+
+    def gt_dsm_utm(self, aoi, resolution=0.5, sub=2) -> np.ndarray:
+        """Per-cell max of the true surface altitude on the UTM grid of a geo
+        aoi.json: each of sub x sub samples per cell goes UTM -> lat/lon ->
+        the altitude where its geodetic vertical meets the surface (three
+        fixed-point rounds of h += H(e, n) - u in the site's ENU frame)."""
+        from . import geodesy
+        lat0, lon0, alt0, zone, south = geodesy.geo_params_from_aoi(aoi)
+        e_size = int(aoi["width"] / resolution) + 1
+        n_size = int(aoi["height"] / resolution) + 1
+        offs = (np.arange(sub) + 0.5) / sub
+        out = np.full((n_size, e_size), -np.inf)
+        cols, rows = np.arange(e_size), np.arange(n_size)
+        for i in range(sub):
+            for j in range(sub):
+                EE, NN = np.meshgrid(aoi["ul_easting"] + (cols + offs[j]) * resolution,
+                                     aoi["ul_northing"] - (rows + offs[i]) * resolution, indexing="xy")
+                lat, lon = geodesy.utm_to_latlon(EE, NN, zone, south == 0)
+                h = np.full(lat.shape, alt0)
+                for _ in range(3):
+                    e, n, u = geodesy.geodetic_to_enu(lat, lon, h, lat0, lon0, alt0)
+                    h = h + (self.true_height(e, n) - u)
+                out = np.maximum(out, h)
+        return out
+
     def gt_dsm(self, resolution=0.5, sub=4) -> np.ndarray:
         """Per-cell max of the true surface on the AOI grid.
 
@@ -609,7 +648,7 @@ def cameras_with_exact_pose(cameras):
 
 
 def build_site(spec: SceneSpec, work_dir: str, device="cpu", save_gt_views=True,
-               log=None) -> dict:
+               log=None, geo=None) -> dict:
     """Render a site and lay it out as the reference pipeline's inputs.
 
     Layout written (``stereo_pipeline.py:396-405`` links ``colmap/mvs`` to it):
@@ -617,6 +656,8 @@ def build_site(spec: SceneSpec, work_dir: str, device="cpu", save_gt_views=True,
       work_dir/colmap/sfm_perspective/images/<name>.png
       work_dir/colmap/sfm_perspective/tri_ba/{cameras,images,points3D}.txt
       work_dir/synthetic_gt/{gt_dsm.npy, <name>.hit_z.npy, <name>.zcam.npy}
+    geo=None writes a local-ENU aoi.json (zone 0); geo=(lat, lon) a UTM one
+    (Scene.geo_aoi) and also synthetic_gt/gt_dsm_utm.npy on its grid.
     """
     from PIL import Image
     scene = Scene(spec, device=device)
@@ -639,10 +680,12 @@ def build_site(spec: SceneSpec, work_dir: str, device="cpu", save_gt_views=True,
     rng = np.random.default_rng(spec.seed_cameras + 17)
     X, tracks = sparse_points(scene, hits, rng)
     write_colmap_text(sparse_dir, scene.cameras, X, tracks)
-    aoi = scene.aoi()
+    aoi = scene.aoi() if geo is None else scene.geo_aoi(*geo)
     with open(os.path.join(work_dir, "aoi.json"), "w") as fp:
         json.dump(aoi, fp, indent=2)
     np.save(os.path.join(gt_dir, "gt_dsm.npy"), scene.gt_dsm(0.5))
+    if geo is not None:
+        np.save(os.path.join(gt_dir, "gt_dsm_utm.npy"), scene.gt_dsm_utm(aoi, 0.5))
     return {"scene": scene, "aoi": aoi, "n_points": len(X)}
 
 

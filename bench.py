@@ -55,13 +55,14 @@ def _device():
     return "cuda" if torch.cuda.is_available() else "cpu"
 
 
-def build_site(spec, work, max_src, src_counts=None):
+def build_site(spec, work, max_src, src_counts=None, dsm_resolution=0.5):
     """Render + dense workspace; src_counts: optional per-view source counts
     written as explicit patch-match.cfg lists."""
     from vissatsatellitestereo_amd import synth, workspace
     if not os.path.exists(os.path.join(work, "colmap", "mvs", "depth_ranges.txt")):
         t = time.time()
-        synth.build_site(spec, work, device=_device(), save_gt_views=False, log=log)
+        synth.build_site(spec, work, device=_device(), save_gt_views=False, log=log,
+                         dsm_resolution=dsm_resolution)
         mvs = workspace.prepare_dense_workspace(work, max_src=max_src)
         if src_counts is not None:
             sel = workspace.select_sources(mvs)
@@ -109,7 +110,7 @@ def run_form(ctx, views, grid, form, iters, use_graph):
         if form == "plain":
             for v in views:
                 ts = time.perf_counter()
-                ctx.solve([v], pass_, want=("depth",))
+                ctx.solve([v], pass_, want=())
                 lat[v["ref"]] = lat.get(v["ref"], 0.0) + time.perf_counter() - ts
         else:
             ctx.solve(views, pass_, want=(), use_graph=use_graph)
@@ -222,21 +223,26 @@ def choose_step(ctx, views, a):
 
 def bench_forms(a, cfg):
     from vissatsatellitestereo_amd import synth, runtime
+    res_m = 0.5
     if cfg == "c2":
         spec, max_src, counts = synth.spec_c2(), 8, None
         desc = "10 reference views x 1024x1024 uint8, 8 sources"
+    elif cfg == "c5":
+        spec, max_src, counts = synth.spec_c5(), 20, None
+        desc = "40 reference views x 4096x4096 uint8, 20 sources, 500 buildings"
+        res_m = 0.3
     else:
         spec = synth.spec_ragged(32)
         rng = np.random.default_rng(7)
         counts = [int(k) for k in rng.integers(6, 13, spec.n_views)]
         max_src = 12
         desc = "32 reference views, ragged crops 512..2048 per side, 6..12 sources"
-    work = build_site(spec, os.path.join(a.work, cfg), max_src, counts)
+    work = build_site(spec, os.path.join(a.work, cfg), max_src, counts, res_m)
     if a.build_only:
         return None
     site = load_site(work)
     gt = np.load(os.path.join(work, "synthetic_gt", "gt_dsm.npy"))
-    grid = runtime.dsm_grid(runtime.read_aoi(work))
+    grid = runtime.dsm_grid(runtime.read_aoi(work), res_m, res_m)
     copy_calibration(a)
     ctx = make_ctx([site], a)
     views = site["views"]
@@ -251,7 +257,7 @@ def bench_forms(a, cfg):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": f"{cfg}: {desc}, photometric+geometric {a.iters}+{a.iters} "
-                               f"iterations, filter, r={a.window_radius}, 0.5 m DSM fusion",
+                               f"iterations, filter, r={a.window_radius}, {res_m} m DSM fusion",
                    "forms": {"plain": "one view per solve call, window step 1",
                              "batched": f"all views per solve call, persistent kernel, hipGraph "
                                         f"replay, window step {step} (autotuner)"},
@@ -357,7 +363,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=("c2", "c3", "c4"))
+    ap.add_argument("--config", default="c2", choices=("c2", "c3", "c4", "c5"))
     ap.add_argument("--iters", type=int, default=5)
     ap.add_argument("--window-radius", type=int, default=5)
     ap.add_argument("--batched-step", default="auto", help="auto (autotuner), 1 or 2")

@@ -21,7 +21,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/vissat_mvs.h"
+#include "vissat_mvs.h"
 #include "vsm_device.h"
 #include "vsm_kernels.h"
 
@@ -72,7 +72,9 @@ struct vsm_ctx {
     uint64_t budget = 0, in_use = 0, peak = 0;
     vsm_params params;
     std::map<int, ImageRec> images;
-    std::map<void *, size_t> allocs;
+    std::map<void *, size_t> allocs;       // live device blocks
+    std::multimap<size_t, void *> cache;   // freed blocks kept for reuse (by size)
+    uint64_t cached_bytes = 0;
     std::string err;
     Counters *d_counters = nullptr;
     unsigned int *d_work = nullptr;        // per-XCD tile cursors of the persistent kernel
@@ -121,15 +123,52 @@ int fail(vsm_ctx *c, int code, const char *fmt, ...)
                         __FILE__, __LINE__);                                                    \
     } while (0)
 
+// Device memory: blocks freed by the library go to a per-context cache and
+// are reused by later allocations of up to the same size (all work runs in
+// order on the context's one stream, so a reused block is never touched by
+// work queued before its reuse).  Live blocks count against the HBM budget;
+// cached blocks are released (hipFree) whenever live + cached + request would
+// exceed it, when hipMalloc fails, and at vsm_destroy.
+constexpr size_t kAllocAlign = 256;
+constexpr uint64_t kCacheMaxBytes = 32ull << 30;
+
+void release_cached(vsm_ctx *c, uint64_t need)
+{
+    while (!c->cache.empty() && (need == UINT64_MAX ||
+                                 c->in_use + c->cached_bytes + need > c->budget)) {
+        auto it = std::prev(c->cache.end());          // largest first
+        (void)hipFree(it->second);
+        c->cached_bytes -= it->first;
+        c->cache.erase(it);
+    }
+}
+
 int dmalloc(vsm_ctx *c, void **p, size_t bytes)
 {
     *p = nullptr;
-    if (bytes == 0) bytes = 16;
+    bytes = (std::max<size_t>(bytes, 1) + kAllocAlign - 1) / kAllocAlign * kAllocAlign;
     if (c->budget && c->in_use + bytes > c->budget)
         return fail(c, VSM_ENOMEM, "HBM budget exceeded: %llu + %llu > %llu bytes",
                     (unsigned long long)c->in_use, (unsigned long long)bytes,
                     (unsigned long long)c->budget);
+    auto hit = c->cache.lower_bound(bytes);
+    if (hit != c->cache.end() && hit->first <= 2 * bytes) {
+        *p = hit->second;
+        const size_t sz = hit->first;
+        c->cached_bytes -= sz;
+        c->cache.erase(hit);
+        c->allocs[*p] = sz;
+        c->in_use += sz;
+        c->peak = std::max(c->peak, c->in_use);
+        return VSM_OK;
+    }
+    if (c->budget) release_cached(c, bytes);
     hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess && !c->cache.empty()) {
+        (void)hipGetLastError();
+        release_cached(c, UINT64_MAX);
+        e = hipMalloc(p, bytes);
+    }
     if (e != hipSuccess) {
         *p = nullptr;
         return fail(c, VSM_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
@@ -144,11 +183,21 @@ void dfree(vsm_ctx *c, void *p)
 {
     if (!p) return;
     auto it = c->allocs.find(p);
-    if (it != c->allocs.end()) {
-        c->in_use -= it->second;
-        c->allocs.erase(it);
+    if (it == c->allocs.end()) {
+        (void)hipFree(p);
+        return;
     }
-    (void)hipFree(p);
+    const size_t sz = it->second;
+    c->in_use -= sz;
+    c->allocs.erase(it);
+    c->cache.emplace(sz, p);
+    c->cached_bytes += sz;
+    while (c->cached_bytes > kCacheMaxBytes && !c->cache.empty()) {
+        auto big = std::prev(c->cache.end());
+        (void)hipFree(big->second);
+        c->cached_bytes -= big->first;
+        c->cache.erase(big);
+    }
 }
 
 template <typename T>
@@ -586,18 +635,20 @@ int run_views(vsm_ctx *c, std::vector<ViewState> &views, const std::vector<const
             RUNCHK(launch_advance(dviews, n, c->stream));
         }
     }
+    // host tile lists stay alive until the stream is synchronised below
+    std::vector<std::vector<Tile>> filter_tiles;
     if (c->params.filter) {
+        filter_tiles.resize(n);
         for (int v = 0; v < n; ++v) {
             // per-view launch so the mask buffer offsets stay simple
-            std::vector<Tile> vt;
+            std::vector<Tile> &vt = filter_tiles[v];
             tiles_for(views[v].desc, 0, vt);
             Tile *dt = nullptr;
             if ((rc = dalloc(c, &dt, vt.size()))) { cleanup(); return rc; }
             RUNCHK(hipMemcpyAsync(dt, vt.data(), vt.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream));
             RUNCHK(launch_filter(maxs, dviews + v, dt, (int)vt.size(), pc,
                                  d_masks ? d_masks + mask_off[v] : nullptr, c->stream));
-            RUNCHK(hipStreamSynchronize(c->stream));
-            dfree(c, dt);
+            dfree(c, dt);                      // back to the cache, stream-ordered
         }
     }
     RUNCHK(hipStreamSynchronize(c->stream));
@@ -629,7 +680,8 @@ int finalize_outputs(vsm_ctx *c, ViewState &vs, int pass, int ref_id, vsm_view_o
         if (d_mask) HIPCHK(c, hipMemcpyAsync(out->consistency, d_mask, hw * 4, hipMemcpyDeviceToHost, c->stream));
         else memset(out->consistency, 0, hw * 4);
     }
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // dv and tmp go back to the cache; the copies above are ordered before
+    // any later use of them on the stream, and the caller synchronises
     dfree(c, dv);
     dfree(c, tmp);
     if (pass == 0) {
@@ -713,6 +765,9 @@ int vsm_destroy(vsm_ctx *c)
     (void)hipStreamSynchronize(c->stream);
     for (auto &kv : c->allocs) (void)hipFree(kv.first);
     c->allocs.clear();
+    for (auto &kv : c->cache) (void)hipFree(kv.second);
+    c->cache.clear();
+    c->cached_bytes = 0;
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -828,6 +883,9 @@ int vsm_solve(vsm_ctx *c, const vsm_view_req *reqs, int32_t n, int32_t pass, vsm
     for (int v = 0; v < n && !rc; ++v)
         rc = finalize_outputs(c, views[v], pass, reqs[v].ref_id, outs ? &outs[v] : nullptr,
                               d_masks ? d_masks + mask_off[v] : nullptr);
+    const hipError_t se = hipStreamSynchronize(c->stream);
+    if (!rc && se != hipSuccess)
+        rc = fail(c, VSM_EDEVICE, "hipStreamSynchronize failed: %s", hipGetErrorString(se));
     for (int v = 0; v < prepared; ++v) release_view(c, views[v]);
     dfree(c, d_masks);
     return rc;

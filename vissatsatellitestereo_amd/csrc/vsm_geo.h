@@ -30,13 +30,9 @@
 // degrees of longitude from the central meridian, is reported invalid).
 #pragma once
 
-#include "../../include/vissat_mvs.h"
+#include "vissat_mvs.h"
 
-#ifdef __HIP_DEVICE_COMPILE__
 #define VSM_GEO_FN __device__ __forceinline__
-#else
-#define VSM_GEO_FN static inline
-#endif
 
 namespace vsm {
 namespace geo {

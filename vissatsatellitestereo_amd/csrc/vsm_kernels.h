@@ -1,7 +1,7 @@
 // vsm_kernels.h -- host-side launchers of the HIP kernels (internal).
 #pragma once
 #include "vsm_device.h"
-#include "../../include/vissat_mvs.h"
+#include "vissat_mvs.h"
 
 namespace vsm {
 

@@ -648,7 +648,7 @@ def cameras_with_exact_pose(cameras):
 
 
 def build_site(spec: SceneSpec, work_dir: str, device="cpu", save_gt_views=True,
-               log=None, geo=None) -> dict:
+               log=None, geo=None, dsm_resolution=0.5) -> dict:
     """Render a site and lay it out as the reference pipeline's inputs.
 
     Layout written (``stereo_pipeline.py:396-405`` links ``colmap/mvs`` to it):
@@ -657,7 +657,8 @@ def build_site(spec: SceneSpec, work_dir: str, device="cpu", save_gt_views=True,
       work_dir/colmap/sfm_perspective/tri_ba/{cameras,images,points3D}.txt
       work_dir/synthetic_gt/{gt_dsm.npy, <name>.hit_z.npy, <name>.zcam.npy}
     geo=None writes a local-ENU aoi.json (zone 0); geo=(lat, lon) a UTM one
-    (Scene.geo_aoi) and also synthetic_gt/gt_dsm_utm.npy on its grid.
+    (Scene.geo_aoi) and also synthetic_gt/gt_dsm_utm.npy on its grid.  The
+    ground-truth DSMs have cells of dsm_resolution metres.
     """
     from PIL import Image
     scene = Scene(spec, device=device)
@@ -683,9 +684,9 @@ def build_site(spec: SceneSpec, work_dir: str, device="cpu", save_gt_views=True,
     aoi = scene.aoi() if geo is None else scene.geo_aoi(*geo)
     with open(os.path.join(work_dir, "aoi.json"), "w") as fp:
         json.dump(aoi, fp, indent=2)
-    np.save(os.path.join(gt_dir, "gt_dsm.npy"), scene.gt_dsm(0.5))
+    np.save(os.path.join(gt_dir, "gt_dsm.npy"), scene.gt_dsm(dsm_resolution))
     if geo is not None:
-        np.save(os.path.join(gt_dir, "gt_dsm_utm.npy"), scene.gt_dsm_utm(aoi, 0.5))
+        np.save(os.path.join(gt_dir, "gt_dsm_utm.npy"), scene.gt_dsm_utm(aoi, dsm_resolution))
     return {"scene": scene, "aoi": aoi, "n_points": len(X)}
 
 

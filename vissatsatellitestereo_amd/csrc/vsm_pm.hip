@@ -115,11 +115,27 @@ __device__ __forceinline__ int cvt_floor(float x)
     return r;
 }
 
+// The fields of a source the window pass reads per tap (copied out of LDS
+// once per evaluation).
+struct SrcView {
+    float lox, hix, loy, hiy;
+    int W, base;
+    const uint32_t *quad;
+};
+
+__device__ __forceinline__ SrcView view_of(const SrcLds &S)
+{
+    SrcView v;
+    v.lox = S.lox; v.hix = S.hix; v.loy = S.loy; v.hiy = S.hiy;
+    v.W = S.W; v.base = S.base; v.quad = S.quad;
+    return v;
+}
+
 // Bilinear sample (DN) at anchor-relative (px, py), already inside the image.
 // The texel index iy*W + ix is formed with a signed 24-bit multiply-add on
 // anchor-relative integers plus the anchor's offset ay*W + ax (|relative
 // coordinates| < 2^23, image side <= 32768).
-__device__ __forceinline__ float sample_quad(const SrcLds &S, float px, float py)
+__device__ __forceinline__ float sample_quad(const SrcView &S, float px, float py)
 {
     const int ix = cvt_floor(px), iy = cvt_floor(py);
     const float fx = __builtin_amdgcn_fractf(px), fy = __builtin_amdgcn_fractf(py);
@@ -178,10 +194,11 @@ constexpr float kDeltaMax = 1e-3f;
 // the G gathers per tap are independent.  EXACT selects the per-tap
 // reciprocal (fallback when some lane's window exceeds kDeltaMax).
 template <int G, bool EXACT, bool CLAMP>
-__device__ __forceinline__ void window_pass(const PMConst &c, const SrcLds &S, const Proj (&P)[G],
-                                            const float (&iz0)[G], const float (&sc)[G],
-                                            const float *s_ref, int lbase, const RefStats &R,
-                                            float (&ss)[G], float (&sss)[G], float (&srs)[G])
+__device__ __forceinline__ void window_pass(const PMConst &c, const SrcView (&SV)[G],
+                                            const Proj (&P)[G], const float (&iz0)[G],
+                                            const float (&sc)[G], const float *s_ref, int lbase,
+                                            const RefStats &R, float (&ss)[G], float (&sss)[G],
+                                            float (&srs)[G])
 {
     const int rad = c.radius, st = c.step, nt = c.ntaps;
     const float frad = (float)(-rad);
@@ -213,6 +230,7 @@ __device__ __forceinline__ void window_pass(const PMConst &c, const SrcLds &S, c
             const float wr = w * r;
 #pragma unroll
             for (int k = 0; k < G; ++k) {
+                const SrcView &S = SV[k];
                 const float qx = fmaf(fi, P[k].hux, rx[k]);
                 const float qy = fmaf(fi, P[k].huy, ry[k]);
                 const float iz = EXACT ? __builtin_amdgcn_rcpf(fmaf(fi, P[k].huz, rz[k]))
@@ -231,22 +249,26 @@ __device__ __forceinline__ void window_pass(const PMConst &c, const SrcLds &S, c
     }
 }
 
-// NCC cost of G plane hypotheses against source S.  Each hypothesis' source
-// samples are centred on its own window-centre sample sc, the reference
-// samples on the reference centre value (NCC is invariant to the offsets;
-// they keep the FP32 moment sums well conditioned when the two views differ
-// in brightness).
+// NCC cost of G plane hypotheses.  Each hypothesis' source samples are
+// centred on its own sample at the window centre, the reference samples on
+// the reference centre value (NCC is invariant to the offsets; they keep the
+// FP32 moment sums well conditioned when the two views differ in brightness).
+// SS[k] is the source of hypothesis k: all the same source (several planes
+// against one source) or G sources of one plane (view selection).
 template <int G>
-__device__ __forceinline__ void ncc_cost_group(const PMConst &c, const SrcLds &S, const Proj (&P)[G],
-                                               const float *s_ref, int lbase, const RefStats &R,
-                                               float (&cost)[G])
+__device__ __forceinline__ void ncc_cost_group(const PMConst &c, const SrcLds *const (&SS)[G],
+                                               const Proj (&P)[G], const float *s_ref, int lbase,
+                                               const RefStats &R, float (&cost)[G])
 {
     bool valid[G];
     float iz0[G], sc[G], ss[G], sss[G], srs[G];
+    SrcView SV[G];
     bool far = false, edge = false;
     const float fr = (float)c.radius;
 #pragma unroll
     for (int k = 0; k < G; ++k) {
+        SV[k] = view_of(*SS[k]);
+        const SrcView &S = SV[k];
         valid[k] = P[k].q0z > 0.f;
         iz0[k] = valid[k] ? __builtin_amdgcn_rcpf(P[k].q0z) : 1.f;
         const float cx = P[k].q0x * iz0[k], cy = P[k].q0y * iz0[k];
@@ -262,11 +284,11 @@ __device__ __forceinline__ void ncc_cost_group(const PMConst &c, const SrcLds &S
         edge = edge || !(cx - mx >= S.lox && cx + mx <= S.hix && cy - my >= S.loy && cy + my <= S.hiy);
     }
     if (__ballot(far) != 0ull)
-        window_pass<G, true, true>(c, S, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
+        window_pass<G, true, true>(c, SV, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
     else if (__ballot(edge) != 0ull)
-        window_pass<G, false, true>(c, S, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
+        window_pass<G, false, true>(c, SV, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
     else
-        window_pass<G, false, false>(c, S, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
+        window_pass<G, false, false>(c, SV, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
 #pragma unroll
     for (int k = 0; k < G; ++k) {
         const float ms = ss[k] * R.inv_sw;
@@ -283,8 +305,9 @@ __device__ __forceinline__ float ncc_cost(const PMConst &c, const SrcLds &S, con
                                           const float *s_ref, int lbase, const RefStats &R)
 {
     const Proj PP[1] = {P};
+    const SrcLds *const SS[1] = {&S};
     float cost[1];
-    ncc_cost_group<1>(c, S, PP, s_ref, lbase, R, cost);
+    ncc_cost_group<1>(c, SS, PP, s_ref, lbase, R, cost);
     return cost[0];
 }
 
@@ -482,13 +505,41 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
             const float j2 = J[2] * nh0[0] + J[5] * nh0[1] + J[8] * nh0[2];
             jtn = sqrtf(j0 * j0 + j1 * j1 + j2 * j2);
         }
+        // costs of the current plane against every source, up to four
+        // sources sharing one window pass
+        for (int s0 = 0; s0 < S;) {
+            const int left = S - s0;
+            if (left >= 7 || left == 4 || left == 5) {
+                Proj P[4];
+                float c4[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) P[k] = make_proj(s_src[s0 + k], du, dv, hd[0], a0, b0);
+                const SrcLds *const SS[4] = {&s_src[s0], &s_src[s0 + 1], &s_src[s0 + 2], &s_src[s0 + 3]};
+                ncc_cost_group<4>(c, SS, P, s_ref, lbase, R, c4);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) s_c0[(s0 + k) * kThreads + tid] = c4[k];
+                s0 += 4;
+            } else if (left == 3 || left == 6) {
+                Proj P[3];
+                float c3[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) P[k] = make_proj(s_src[s0 + k], du, dv, hd[0], a0, b0);
+                const SrcLds *const SS[3] = {&s_src[s0], &s_src[s0 + 1], &s_src[s0 + 2]};
+                ncc_cost_group<3>(c, SS, P, s_ref, lbase, R, c3);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) s_c0[(s0 + k) * kThreads + tid] = c3[k];
+                s0 += 3;
+            } else {
+                const Proj P = make_proj(s_src[s0], du, dv, hd[0], a0, b0);
+                s_c0[s0 * kThreads + tid] = ncc_cost(c, s_src[s0], P, s_ref, lbase, R);
+                s0 += 1;
+            }
+        }
         float tot = 0.f;
         int lastpos = 0;
         for (int s = 0; s < S; ++s) {
             const SrcLds &Ss = s_src[s];
-            const Proj P = make_proj(Ss, du, dv, hd[0], a0, b0);
-            const float c0 = ncc_cost(c, Ss, P, s_ref, lbase, R);
-            s_c0[s * kThreads + tid] = c0;
+            const float c0 = s_c0[s * kThreads + tid];
             float prev = 0.f;
             int np = 0;
 #pragma unroll
@@ -581,7 +632,8 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
                 float ph[kGroupA];
 #pragma unroll
                 for (int k = 0; k < kGroupA; ++k) P[k] = make_proj(Ss, du, dv, hd[k + 1], ak[k], bk[k]);
-                ncc_cost_group<kGroupA>(c, Ss, P, s_ref, lbase, R, ph);
+                const SrcLds *const SS[kGroupA] = {&Ss, &Ss, &Ss, &Ss};
+                ncc_cost_group<kGroupA>(c, SS, P, s_ref, lbase, R, ph);
 #pragma unroll
                 for (int k = 0; k < kGroupA; ++k)
                     acc[k] += cnt * total_cost(c, Ss, P[k], ph[k], pass, du, dv);
@@ -593,7 +645,8 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
 #pragma unroll
                 for (int k = 0; k < GB; ++k)
                     P[k] = make_proj(Ss, du, dv, hd[kGroupA + 1 + k], ak[kGroupA + k], bk[kGroupA + k]);
-                ncc_cost_group<GB>(c, Ss, P, s_ref, lbase, R, ph);
+                const SrcLds *const SS[GB] = {&Ss, &Ss, &Ss};
+                ncc_cost_group<GB>(c, SS, P, s_ref, lbase, R, ph);
 #pragma unroll
                 for (int k = 0; k < GB; ++k)
                     acc[kGroupA + k] += cnt * total_cost(c, Ss, P[k], ph[k], pass, du, dv);

@@ -684,7 +684,9 @@ def build_site(spec: SceneSpec, work_dir: str, device="cpu", save_gt_views=True,
     aoi = scene.aoi() if geo is None else scene.geo_aoi(*geo)
     with open(os.path.join(work_dir, "aoi.json"), "w") as fp:
         json.dump(aoi, fp, indent=2)
-    np.save(os.path.join(gt_dir, "gt_dsm.npy"), scene.gt_dsm(dsm_resolution))
+    # sub-cell sampling: 4 x 4 samples per 0.5 m cell, 2 x 2 for finer grids
+    np.save(os.path.join(gt_dir, "gt_dsm.npy"),
+            scene.gt_dsm(dsm_resolution, sub=4 if dsm_resolution >= 0.5 else 2))
     if geo is not None:
         np.save(os.path.join(gt_dir, "gt_dsm_utm.npy"), scene.gt_dsm_utm(aoi, dsm_resolution))
     return {"scene": scene, "aoi": aoi, "n_points": len(X)}

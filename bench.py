@@ -99,13 +99,16 @@ def dsm_error(dsm, gt):
 # ---------------------------------------------------------------------------
 # plain vs batched
 # ---------------------------------------------------------------------------
-def run_form(ctx, views, grid, form, iters, use_graph):
-    """One step: photometric + geometric for every view, then fusion.
-    Returns (wall seconds, per-view latencies, fused DSM)."""
+PASSES_FULL = ((0, dict(geom_consistency=0, filter=0)), (1, dict(geom_consistency=1, filter=1)))
+PASSES_PHOTO = ((0, dict(geom_consistency=0, filter=0)),)
+
+
+def run_form(ctx, views, grid, form, iters, use_graph, photometric_only=False):
+    """One step: photometric (+ geometric with filter) for every view, then
+    fusion.  Returns (wall seconds, per-view latencies, fused DSM)."""
     lat = {}
     t0 = time.perf_counter()
-    for pass_, extra in ((0, dict(geom_consistency=0, filter=0)),
-                         (1, dict(geom_consistency=1, filter=1))):
+    for pass_, extra in (PASSES_PHOTO if photometric_only else PASSES_FULL):
         ctx.set_params(num_iterations=iters, **extra)
         if form == "plain":
             for v in views:
@@ -114,7 +117,9 @@ def run_form(ctx, views, grid, form, iters, use_graph):
                 lat[v["ref"]] = lat.get(v["ref"], 0.0) + time.perf_counter() - ts
         else:
             ctx.solve(views, pass_, want=(), use_graph=use_graph)
-    dsm = ctx.fuse_dsm(grid, [v["ref"] for v in views], depth_kind=1, min_measurements=3)
+    # two views (c1) cannot pass the reference's <= 2 rejection: keep every cell there
+    dsm = ctx.fuse_dsm(grid, [v["ref"] for v in views], depth_kind=0 if photometric_only else 1,
+                       min_measurements=1 if photometric_only else 3)
     wall = time.perf_counter() - t0
     if form != "plain":
         lat = {v["ref"]: wall for v in views}
@@ -126,15 +131,16 @@ def measure_forms(ctx, views, grid, gt, a, batched_step):
     for form in [f for f in ("plain", "batched") if f in a.forms.split(",")]:
         ctx.set_params(window_step=1 if form == "plain" else batched_step)
         use_graph = form == "batched"
+        po = getattr(a, "photometric_only", False)
         for _ in range(a.warmup):
-            run_form(ctx, views, grid, form, a.iters, use_graph)
+            run_form(ctx, views, grid, form, a.iters, use_graph, po)
         log(f"{form}: warm")
         ctx.reset_stats()
         ctx.set_timing(form == "batched")
         walls, lats = [], []
         ctx.synchronize()
         for k in range(a.steps):
-            w, lat, dsm = run_form(ctx, views, grid, form, a.iters, use_graph)
+            w, lat, dsm = run_form(ctx, views, grid, form, a.iters, use_graph, po)
             walls.append(w)
             lats += lat
             log(f"{form} step {k}: {w * 1e3:.0f} ms")
@@ -200,6 +206,42 @@ def cpu_baseline(site, window_radius, iters):
                       f"sample x {scale:.0f} (area) x {2 * iters} (iterations of both passes)"}
 
 
+def cpu_baseline_full(site, window_radius, iters):
+    """Oracle (FP64 C, OpenMP) on the whole c1 workload: every view, all
+    iterations, no extrapolation."""
+    import pm_oracle as po
+    inv_ids = {i: n for n, i in site["ids"].items()}
+    imgs, P4 = site["imgs"], site["P4"]
+    t = time.perf_counter()
+    for v in site["views"]:
+        ref = inv_ids[v["ref"]]
+        srcs = [inv_ids[s] for s in v["srcs"]]
+        ov = po.OracleView(imgs[ref], [imgs[s] for s in srcs], P4[ref], [P4[s] for s in srcs],
+                           v["depth_range"], view_id=v["ref"], params=dict(window_radius=window_radius))
+        ov.solve(0, iters)
+    dt = time.perf_counter() - t
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    n = len(site["views"])
+    return {"value": round(n / dt, 4), "unit": "views/s", "cores": cores, "kind": "port",
+            "sample": f"the whole workload: {n} views, {iters} photometric iterations, in {dt:.2f}s"}
+
+
+def cpu_baseline_fusion(ctx, site, grid):
+    """numpy restatement of aggregate_2p5d (oracle/fusion_oracle.py), one
+    process, on the geometric depth maps of the last batched step."""
+    import fusion_oracle as fo
+    ids = [v["ref"] for v in site["views"]]
+    depths = [ctx.get_depth_map(i, kind=1) for i in ids]
+    invs = [np.linalg.inv(site["P4"][n]) for n in site["names"]]
+    t = time.perf_counter()
+    stack = np.stack([fo.view_dsm(d, inv, grid) for d, inv in zip(depths, invs)])
+    fo.fuse(stack, 3)
+    dt = time.perf_counter() - t
+    return {"value": round(len(ids) / dt, 4), "unit": "views/s (fusion only)", "cores": 1,
+            "kind": "port", "sample": f"all {len(ids)} geometric depth maps -> per-view DSMs -> "
+                                      f"fused DSM in {dt:.2f}s"}
+
+
 def make_ctx(sites, a):
     from vissatsatellitestereo_amd.mvs import MvsContext
     ctx = MvsContext(device=0, hbm_budget_bytes=int(a.hbm_budget_gb * 1e9),
@@ -224,7 +266,11 @@ def choose_step(ctx, views, a):
 def bench_forms(a, cfg):
     from vissatsatellitestereo_amd import synth, runtime
     res_m = 0.5
-    if cfg == "c2":
+    a.photometric_only = cfg == "c1"
+    if cfg == "c1":
+        spec, max_src, counts = synth.spec_c1(), 1, None
+        desc = "2 views x 256x256 uint8, 1 source each, photometric only"
+    elif cfg == "c2":
         spec, max_src, counts = synth.spec_c2(), 8, None
         desc = "10 reference views x 1024x1024 uint8, 8 sources"
     elif cfg == "c5":
@@ -250,7 +296,14 @@ def bench_forms(a, cfg):
     res = measure_forms(ctx, views, grid, gt, a, step)
     b = res.get("batched") or res["plain"]
     p = res.get("plain") or b
-    cpu = None if a.no_cpu_baseline else cpu_baseline(site, a.window_radius, a.iters)
+    if a.no_cpu_baseline:
+        cpu = cpu_fuse = None
+    elif cfg == "c1":
+        cpu = cpu_baseline_full(site, a.window_radius, a.iters)
+        cpu_fuse = None
+    else:
+        cpu = cpu_baseline(site, a.window_radius, a.iters)
+        cpu_fuse = cpu_baseline_fusion(ctx, site, grid)
     out = {
         "metric": METRIC, "value": round(b["views_per_s"], 4), "unit": "views/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(b["ms_per_step"], 2),
@@ -271,7 +324,13 @@ def bench_forms(a, cfg):
                                  "coverage_batched": round(b["dsm_cov"], 4)},
         "roofline": roofline(b["stats"]),
         "cpu_baseline": cpu,
+        "fusion": {"gpu_ms_per_step": round(b["stats"]["fuse_ms"] / max(1, a.steps), 3)
+                   if b["stats"]["fuse_ms"] else None,
+                   "cpu_baseline": cpu_fuse},
     }
+    if a.photometric_only:
+        out["config"]["workload"] = (f"{cfg}: {desc}, {a.iters} iterations, r={a.window_radius}, "
+                                     f"{res_m} m DSM fusion of the photometric maps (every cell kept)")
     ctx.close()
     return out
 
@@ -343,18 +402,20 @@ def bench_serving(a):
 
 
 def copy_calibration(a):
-    """Optional: a device copy of known size (HBM counter calibration); a
-    512 MiB flush buffer written in between pushes the source out of L2/MALL."""
+    """Optional HBM counter calibration: one elementwise kernel that reads and
+    writes a_mb MiB each (dst = src * 1), after a flush buffer larger than the
+    last-level cache has been written, so its reads come from HBM."""
     if a.copy_calibration_mb <= 0:
         return
     import torch
     n = a.copy_calibration_mb * (1 << 20) // 4
     src = torch.ones(n, dtype=torch.float32, device="cuda")
-    flush = torch.ones(max(n, (512 << 20) // 4), dtype=torch.float32, device="cuda")
+    flush = torch.ones(max(n, (1024 << 20) // 4), dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
-    dst.copy_(src)
     torch.cuda.synchronize()
-    log(f"copy calibration: {a.copy_calibration_mb} MiB read + written")
+    torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    log(f"copy calibration: {a.copy_calibration_mb} MiB read + written by one elementwise kernel")
     del src, dst, flush
 
 
@@ -363,7 +424,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=("c2", "c3", "c4", "c5"))
+    ap.add_argument("--config", default="c2", choices=("c1", "c2", "c3", "c4", "c5"))
     ap.add_argument("--iters", type=int, default=5)
     ap.add_argument("--window-radius", type=int, default=5)
     ap.add_argument("--batched-step", default="auto", help="auto (autotuner), 1 or 2")

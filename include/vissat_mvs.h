@@ -188,6 +188,14 @@ int vsm_remove_image(vsm_ctx *ctx, int32_t image_id);
 int vsm_set_depth_map(vsm_ctx *ctx, int32_t image_id, int32_t kind, const float *depth, int32_t w,
                       int32_t h);
 
+/* Copy the depth map of kind 0 (photometric) or 1 (geometric) the context
+ * holds for image_id into a caller-owned w*h float buffer: the map a solve
+ * left on the device, i.e. what depth_maps/<img>.<kind>.bin holds after the
+ * reference's patch_match_stereo run (colmap_mvs_commands.py:39-82).
+ * VSM_ENOENT if there is none yet. */
+int vsm_get_depth_map(vsm_ctx *ctx, int32_t image_id, int32_t kind, float *depth, int32_t w,
+                      int32_t h);
+
 /* Solve n reference views in one batched launch sequence and block until
  * done.  pass 0 = photometric (run_photometric_mvs,
  * colmap_mvs_commands.py:39-55), pass 1 = geometric consistency + filter

@@ -302,3 +302,26 @@ def test_filter_matches_oracle(tiny):
     keep_g = g["depth"] > 0
     keep_o = so["depth"] > 0
     assert np.mean(keep_g != keep_o) <= 1e-3
+
+
+def test_get_depth_map_round_trip(tiny):
+    """vsm_get_depth_map returns what vsm_set_depth_map stored and the solve's
+    own depth output; unknown kinds / missing maps raise."""
+    from vissatsatellitestereo_amd._lib import VsmError
+    from vissatsatellitestereo_amd.mvs import MvsContext
+    ctx = MvsContext(device=0, window_radius=3, num_iterations=2)
+    try:
+        names = tiny["names"]
+        for i, n in enumerate(names):
+            ctx.add_image(700 + i, tiny["imgs"][n], tiny["P4"][n])
+        with pytest.raises(VsmError):
+            ctx.get_depth_map(700, kind=0)
+        rng = np.random.default_rng(4)
+        d = rng.uniform(1, 5, tiny["imgs"][names[0]].shape).astype(np.float32)
+        ctx.set_depth_map(700, d, kind=1)
+        assert np.array_equal(ctx.get_depth_map(700, kind=1), d)
+        v = dict(ref=701, srcs=[700, 702], depth_range=tiny["dr"][names[1]])
+        out = ctx.solve([v], 0, want=("depth",))
+        assert np.array_equal(ctx.get_depth_map(701, kind=0), out[0]["depth"])
+    finally:
+        ctx.close()

@@ -193,3 +193,28 @@ def test_no_device_raises_loudly():
     from vissatsatellitestereo_amd._lib import VsmError
     with pytest.raises(VsmError):
         MvsContext(device=0)
+
+
+def test_consistency_graph_records_round_trip(tmp_path):
+    """Bitmask -> COLMAP consistency-graph records -> back, against a
+    per-pixel loop."""
+    from vissatsatellitestereo_amd.colmap import consistency_graph as cg
+    rng = np.random.default_rng(9)
+    mask = rng.integers(0, 1 << 5, (7, 11)).astype(np.uint32)
+    mask[rng.random(mask.shape) < 0.4] = 0
+    src = [4, 0, 9, 2, 7]
+    path = str(tmp_path / "g.bin")
+    cg.write(path, mask, src)
+    got = cg.read(path)
+    exp = {}
+    for r in range(7):
+        for c in range(11):
+            ids = [src[b] for b in range(5) if (int(mask[r, c]) >> b) & 1]
+            if ids:
+                exp[(r, c)] = ids
+    assert got == exp
+    assert list(got) == sorted(got)          # row-major record order
+    raw = np.fromfile(path, dtype="<i4")
+    assert raw.size == sum(3 + len(v) for v in exp.values())
+    cg.write(path, np.zeros((3, 3), np.uint32), src)
+    assert cg.read(path) == {}

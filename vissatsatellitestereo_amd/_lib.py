@@ -110,6 +110,8 @@ SIGNATURES = [
     ("vsm_remove_image", _c.c_int, [_c.c_void_p, _c.c_int32]),
     ("vsm_set_depth_map", _c.c_int, [_c.c_void_p, _c.c_int32, _c.c_int32, P(_c.c_float), _c.c_int32,
                                      _c.c_int32]),
+    ("vsm_get_depth_map", _c.c_int, [_c.c_void_p, _c.c_int32, _c.c_int32, P(_c.c_float), _c.c_int32,
+                                     _c.c_int32]),
     ("vsm_solve", _c.c_int, [_c.c_void_p, P(ViewReq), _c.c_int32, _c.c_int32, P(ViewOut), _c.c_int32]),
     ("vsm_submit", _c.c_int, [_c.c_void_p, P(ViewReq), P(_c.c_uint64)]),
     ("vsm_serve_step", _c.c_int, [_c.c_void_p, P(_c.c_int32), P(_c.c_int32)]),

@@ -11,8 +11,10 @@ read and write:
 * out: stereo/depth_maps/<img>.{photometric,geometric}.bin,
        stereo/normal_maps/<img>.{photometric,geometric}.bin (COLMAP array
        format, colmap/read_dense.py:36-51), stereo/consistency_graphs/
-       <img>.geometric.npy (uint32 bitmask of consistent sources per pixel),
-       and inv_proj_mats.txt (aggregate_2p5d_util.py:55-61).
+       <img>.geometric.bin (COLMAP consistency-graph records, see
+       colmap/consistency_graph.py) and <img>.geometric.npy (the same as a
+       uint32 bitmask of consistent sources per pixel), and inv_proj_mats.txt
+       (aggregate_2p5d_util.py:55-61).
 
 Instead of spawning `colmap patch_match_stereo` (lib/run_cmd.py:44) the
 views are solved in-process by the HIP library.  Errors raise (the
@@ -29,6 +31,7 @@ import os
 
 import numpy as np
 
+from .colmap import consistency_graph
 from .colmap.read_dense import write_array
 from . import runtime
 from .workspace import select_sources
@@ -117,4 +120,7 @@ def run_consistency_check(mvs_dir, window_radius, depth_range=None, num_iteratio
         write_array(r["depth"], ws.depth_path(ref, "geometric"))
         write_array(r["normal"], ws.normal_path(ref, "geometric"))
         np.save(os.path.join(cg, f"{ref}.geometric.npy"), r["consistency"])
+        srcs = next(v[1] for v in views if v[0] == ref)
+        consistency_graph.write(os.path.join(cg, f"{ref}.geometric.bin"), r["consistency"],
+                                [ws.names.index(sn) for sn in srcs])
     return res

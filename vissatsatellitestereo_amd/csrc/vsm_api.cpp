@@ -856,6 +856,21 @@ int vsm_set_depth_map(vsm_ctx *c, int32_t id, int32_t kind, const float *depth, 
     return VSM_OK;
 }
 
+int vsm_get_depth_map(vsm_ctx *c, int32_t id, int32_t kind, float *depth, int32_t w, int32_t h)
+{
+    if (!c || !depth) return VSM_EINVAL;
+    if (kind != 0 && kind != 1) return fail(c, VSM_EINVAL, "depth map kind must be 0 or 1");
+    auto it = c->images.find(id);
+    if (it == c->images.end()) return fail(c, VSM_ENOENT, "unknown image %d", id);
+    const ImageRec &r = it->second;
+    if (w != r.W || h != r.H) return fail(c, VSM_EINVAL, "buffer %dx%d != image %dx%d", w, h, r.W, r.H);
+    if (!r.depth[kind]) return fail(c, VSM_ENOENT, "image %d has no depth map of kind %d", id, kind);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(depth, r.depth[kind], (size_t)w * h * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VSM_OK;
+}
+
 int vsm_solve(vsm_ctx *c, const vsm_view_req *reqs, int32_t n, int32_t pass, vsm_view_out *outs,
               int32_t use_graph)
 {

@@ -90,6 +90,13 @@ class MvsContext:
         self._check(self.lib.vsm_remove_image(self.h, int(image_id)))
         self.sizes.pop(int(image_id), None)
 
+    def get_depth_map(self, image_id, kind=1):
+        """The depth map (kind 0 photometric, 1 geometric) held on the device."""
+        h, w = self.sizes[int(image_id)]
+        out = np.zeros((h, w), np.float32)
+        self._check(self.lib.vsm_get_depth_map(self.h, int(image_id), int(kind), fptr(out), w, h))
+        return out
+
     def set_depth_map(self, image_id, depth, kind=0):
         d = np.ascontiguousarray(depth, dtype=np.float32)
         h, w = d.shape

@@ -157,14 +157,16 @@ def measure_forms(ctx, views, grid, gt, a, batched_step):
     return res
 
 
-def roofline(st):
+def roofline(st, cfg="c2"):
     if not (st["halfstep_ms"] > 0 and st["halfstep_launches"] > 0):
         return None
     avg_ms = st["halfstep_ms"] / st["halfstep_launches"]
     taps = st["taps"] / st["halfstep_launches"]
     ach = FLOPS_PER_TAP * taps / (avg_ms * 1e-3) / 1e12
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "k_halfstep_c2_counters.json")
+    # HBM traffic per launch comes from a committed counter profile of the
+    # same configuration (none for the others)
+    tfile = os.path.join(ROOT, "profiles", f"k_halfstep_{cfg}_counters.json")
     if os.path.exists(tfile):
         t = json.load(open(tfile))
         traffic = {"bytes_per_launch": round(t["fetch_bytes"] + t["write_bytes"]),
@@ -337,7 +339,7 @@ def bench_forms(a, cfg):
         "dsm_median_abs_err_m": {"plain": p["dsm_err"], "batched": b["dsm_err"],
                                  "coverage_plain": round(p["dsm_cov"], 4),
                                  "coverage_batched": round(b["dsm_cov"], 4)},
-        "roofline": roofline(b["stats"]),
+        "roofline": roofline(b["stats"], cfg),
         "cpu_baseline": cpu,
         "fusion": {"gpu_ms_per_step": round(b["stats"]["fuse_ms"] / max(1, a.steps), 3)
                    if b["stats"]["fuse_ms"] else None,
@@ -365,25 +367,54 @@ def bench_serving(a):
         works.append(build_site(spec, os.path.join(a.work, f"c4_site{k}"), 11, counts))
     if a.build_only:
         return None
+    from vissatsatellitestereo_amd import runtime
     sites = [load_site(w, id_base=1000 * (k + 1)) for k, w in enumerate(works)]
+    grids = [runtime.dsm_grid(runtime.read_aoi(w)) for w in works]
+    gts = [np.load(os.path.join(w, "synthetic_gt", "gt_dsm.npy")) for w in works]
+    site_of = {v["ref"]: k for k, s in enumerate(sites) for v in s["views"]}
     copy_calibration(a)
     ctx = make_ctx(sites, a)
-    ctx.set_params(num_iterations=a.iters, geom_consistency=0, filter=1,
-                   window_step=1 if a.batched_step == "auto" else int(a.batched_step))
     views = [v for s in sites for v in s["views"]]
+    ctx.set_params(num_iterations=a.iters, geom_consistency=0, filter=1)
+    step, tune_rep = choose_step(ctx, views, a)
     rng = np.random.default_rng(6)
     order = rng.permutation(len(views))
     reqs = [views[i] for i in order][:a.requests]
     arrivals = poisson_arrivals(len(reqs), a.rate, seed=6)
+
+    # incremental per-site DSM: each finished request re-fuses its site from
+    # the photometric (filtered) depth maps of the site's finished requests
+    fused = {"updates": 0, "ms": 0.0}
+    finished = {k: [] for k in range(len(sites))}
+    dsm = {}
+
+    def on_complete(ticket, view, result):
+        k = site_of[view["ref"]]
+        finished[k].append(view["ref"])
+        if len(finished[k]) >= 3:
+            t = time.perf_counter()
+            dsm[k] = ctx.fuse_dsm(grids[k], finished[k], depth_kind=0, min_measurements=3)
+            fused["ms"] += 1e3 * (time.perf_counter() - t)
+            fused["updates"] += 1
+
+    def site_errors():
+        errs = [dsm_error(dsm[k], gts[k])[0] for k in sorted(dsm)]
+        errs = [e for e in errs if e is not None]
+        return round(float(np.median(errs)), 4) if errs else None
+
+    ctx.set_params(num_iterations=a.iters, geom_consistency=0, filter=1, window_step=step)
     warm = ServeLoop(ctx)                    # one request end to end
     warm.submit(reqs[0])
     warm.drain()
-    loop = ServeLoop(ctx)
+    loop = ServeLoop(ctx, on_complete=on_complete)
     ctx.reset_stats()
     served = loop.run_schedule(arrivals, reqs)
     st = ctx.stats()
-    log("served", {k: v for k, v in served.items() if k not in ("tickets", "latency_s")})
-    # one-request-at-a-time queue on the same arrival schedule
+    served_err = site_errors()
+    log("served", {k: v for k, v in served.items() if k not in ("tickets", "latency_s")},
+        "dsm", served_err, fused)
+    # one-request-at-a-time queue on the same arrival schedule (plain form)
+    ctx.set_params(window_step=1)
     t0 = time.perf_counter()
     lat = []
     for t_arr, v in zip(arrivals, reqs):
@@ -393,7 +424,10 @@ def bench_serving(a):
         ctx.solve([v], 0, want=("depth",))
         lat.append(time.perf_counter() - t0 - t_arr)
     plain_wall = time.perf_counter() - t0
-    log(f"queue: {len(reqs) / plain_wall:.2f} views/s")
+    for k in list(dsm):
+        dsm[k] = ctx.fuse_dsm(grids[k], finished[k], depth_kind=0, min_measurements=3)
+    queue_err = site_errors()
+    log(f"queue: {len(reqs) / plain_wall:.2f} views/s", "dsm", queue_err)
     out = {
         "metric": METRIC, "value": round(served["views_per_s"], 4), "unit": "views/s", "n_gpus": 1,
         "steps": len(reqs), "warmup": 1, "ms_per_step": round(1e3 * served["wall_s"] / len(reqs), 2),
@@ -402,14 +436,21 @@ def bench_serving(a):
         "config": {"workload": f"c4 serving: {len(reqs)} per-view requests, Poisson {a.rate} req/s, "
                                f"4 sites, crops 1024..{a.serve_max_side} per side, 8..11 sources, "
                                f"photometric {a.iters} iterations + filter, r={a.window_radius}, "
-                               f"HBM budget {a.hbm_budget_gb} GB"},
+                               f"HBM budget {a.hbm_budget_gb} GB, incremental per-site 0.5 m DSM",
+                   "forms": {"plain": "one request at a time in arrival order, window step 1",
+                             "batched": f"continuous batching (vsm_submit / vsm_serve_step), "
+                                        f"window step {step} (autotuner)"},
+                   "autotune": tune_rep},
+        "dsm_median_abs_err_m": {"plain": queue_err, "batched": served_err,
+                                 "note": "median over sites of each site's median |dh|"},
+        "incremental_fusion": {"updates": fused["updates"], "ms_total": round(fused["ms"], 1)},
         "latency_ms": {"served_p50": round(served["p50_ms"], 1), "served_p99": round(served["p99_ms"], 1),
                        "queue_p50": round(1e3 * float(np.percentile(lat, 50)), 1),
                        "queue_p99": round(1e3 * float(np.percentile(lat, 99)), 1)},
         "plain_views_per_s": round(len(reqs) / plain_wall, 4),
         "batched_over_plain": round(served["views_per_s"] / (len(reqs) / plain_wall), 3),
         "serve_steps": served["steps"],
-        "roofline": roofline(st),
+        "roofline": roofline(st, "c4"),
         "cpu_baseline": None,
     }
     ctx.close()
@@ -449,7 +490,7 @@ def main():
     ap.add_argument("--build-only", action="store_true", help="build the synthetic site(s) and exit")
     ap.add_argument("--copy-calibration-mb", type=int, default=0)
     ap.add_argument("--hbm-budget-gb", type=float, default=200.0)
-    ap.add_argument("--rate", type=float, default=10.0, help="c4 arrival rate (req/s)")
+    ap.add_argument("--rate", type=float, default=20.0, help="c4 arrival rate (req/s)")
     ap.add_argument("--requests", type=int, default=40, help="c4 number of requests")
     ap.add_argument("--serve-max-side", type=int, default=2048)
     ap.add_argument("--work", default=os.environ.get("VSM_BENCH_WORK", "/tmp/vsm_bench"))

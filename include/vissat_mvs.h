@@ -212,6 +212,9 @@ int vsm_solve(vsm_ctx *ctx, const vsm_view_req *reqs, int32_t n, int32_t pass,
  * (photometric consistency only: NCC and triangulation angle).  A request's
  * result does not depend on which other requests share its launches.
  * vsm_poll copies the result into out when done (*done = 1), else *done = 0.
+ * A finished request's depth also becomes its image's photometric depth map
+ * on the device (vsm_get_depth_map kind 0, vsm_fuse_dsm depth_kind 0), as
+ * after vsm_solve pass 0.
  * vsm_serve_step admits what fits and advances every admitted request by one
  * half-step (one batched launch). */
 int vsm_submit(vsm_ctx *ctx, const vsm_view_req *req, uint64_t *ticket);

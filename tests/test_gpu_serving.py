@@ -100,3 +100,28 @@ def test_autotune_picks_and_caches(tiny, tmp_path):
     st2, rep2 = autotune.tune(ctx, views, iters=3, radius=4, cache_path=cache)
     assert st2 == st and rep2["cached"]
     ctx.close()
+
+
+def test_served_result_becomes_device_depth_and_fuses(tiny):
+    """A finished request's depth is the image's photometric depth map on the
+    device (what incremental per-site fusion reads), and the on_complete hook
+    sees every request once."""
+    from vissatsatellitestereo_amd.serving import ServeLoop
+    views = _views(tiny)
+    ctx = _ctx(tiny, filter=1)
+    seen = []
+    loop = ServeLoop(ctx, want=("depth",), on_complete=lambda t, v, r: seen.append((t, v["ref"])))
+    tickets = [loop.submit(v) for v in views]
+    loop.drain()
+    assert sorted(t for t, _ in seen) == sorted(tickets)
+    for t, v in zip(tickets, views):
+        assert np.array_equal(ctx.get_depth_map(v["ref"], kind=0), loop.result(t)["depth"])
+    import json, os
+    import fusion_oracle as fo
+    grid = fo.grid_from_aoi(json.load(open(os.path.join(tiny["wd"], "aoi.json"))))
+    ids = [v["ref"] for v in views]
+    dsm = ctx.fuse_dsm(grid, ids, depth_kind=0, min_measurements=3)
+    views_o = [fo.view_dsm(loop.result(t)["depth"], np.linalg.inv(tiny["P4"][n]), grid)
+               for t, n in zip(tickets, tiny["names"])]
+    assert np.array_equal(dsm, fo.fuse(np.stack(views_o), 3), equal_nan=True)
+    ctx.close()

@@ -36,6 +36,8 @@ struct ImageRec {
     double P4[16];
     double Pinv[16];
     float *depth[2] = {nullptr, nullptr};   // photometric / geometric depth kept on device
+    float4 *depth4 = nullptr;               // photometric depth, 2x2 neighbourhood per texel
+    uint64_t depth_ver = 0, depth4_ver = 0; // depth4 is current when the versions agree
     float4 *hyp_photo = nullptr;            // photometric hypotheses (geometric init)
 };
 
@@ -334,7 +336,7 @@ int build_geometry(vsm_ctx *c, const ImageRec &ref, const std::vector<const Imag
         V.src_w[s] = S.W;
         V.src_h[s] = S.H;
         V.src_quad[s] = S.quad;
-        V.src_depth[s] = nullptr;
+        V.src_depth4[s] = nullptr;
     }
     // Rotation of the reference camera from P[:3,:3] = K R.
     const double *A = ref.P4;
@@ -439,7 +441,7 @@ int prepare_view(vsm_ctx *c, const vsm_view_req &r, int pass, uint32_t view_id, 
     if (r.n_src < 1 || r.n_src > VSM_MAX_SRC)
         return fail(c, VSM_EINVAL, "n_src %d outside [1, %d]", r.n_src, VSM_MAX_SRC);
     if (!r.src_ids) return fail(c, VSM_EINVAL, "src_ids is NULL");
-    std::vector<const ImageRec *> srcs;
+    std::vector<ImageRec *> srcs;
     for (int s = 0; s < r.n_src; ++s) {
         auto si = c->images.find(r.src_ids[s]);
         if (si == c->images.end()) return fail(c, VSM_ENOENT, "unknown source image %d", r.src_ids[s]);
@@ -453,7 +455,8 @@ int prepare_view(vsm_ctx *c, const vsm_view_req &r, int pass, uint32_t view_id, 
         return fail(c, VSM_EINVAL, "invalid depth range [%g, %g] for image %d", dmin, dmax, r.ref_id);
     memset(&vs.desc, 0, sizeof(ViewDesc));
     ViewDesc &V = vs.desc;
-    int rc = build_geometry(c, ref, srcs, dmin, dmax, V);
+    int rc = build_geometry(c, ref, std::vector<const ImageRec *>(srcs.begin(), srcs.end()), dmin,
+                            dmax, V);
     if (rc) return rc;
     V.W = ref.W;
     V.H = ref.H;
@@ -466,12 +469,17 @@ int prepare_view(vsm_ctx *c, const vsm_view_req &r, int pass, uint32_t view_id, 
     V.ref = ref.gray;
     if (pass == 1) {
         for (int s = 0; s < r.n_src; ++s) {
-            const ImageRec &S = *srcs[s];
+            ImageRec &S = *srcs[s];
             if (!S.depth[0])
                 return fail(c, VSM_EINVAL,
                             "geometric pass needs the photometric depth map of source image %d",
                             S.id);
-            V.src_depth[s] = S.depth[0];
+            if (!S.depth4 || S.depth4_ver != S.depth_ver) {
+                if (!S.depth4 && (rc = dalloc(c, &S.depth4, (size_t)S.W * S.H))) return rc;
+                HIPCHK(c, launch_pack_depth4(S.depth[0], S.W, S.H, S.depth4, c->stream));
+                S.depth4_ver = S.depth_ver;
+            }
+            V.src_depth4[s] = S.depth4;
         }
     }
     const size_t hw = (size_t)ref.W * ref.H;
@@ -670,6 +678,7 @@ int finalize_outputs(vsm_ctx *c, ViewState &vs, int pass, int ref_id, vsm_view_o
     if (!ref.depth[pass] && (rc = dalloc(c, &ref.depth[pass], hw))) { dfree(c, dv); dfree(c, tmp); return rc; }
     HIPCHK(c, hipMemcpyAsync(dv, &vs.desc, sizeof(ViewDesc), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, launch_finalize(dv, 0, (int)hw, tmp, tmp + hw, tmp + 4 * hw, ref.depth[pass], c->stream));
+    if (pass == 0) ++ref.depth_ver;
     if (out && out->depth)
         HIPCHK(c, hipMemcpyAsync(out->depth, tmp, hw * 4, hipMemcpyDeviceToHost, c->stream));
     if (out && out->normal)
@@ -834,6 +843,7 @@ int vsm_remove_image(vsm_ctx *c, int32_t id)
     dfree(c, r.quad);
     dfree(c, r.depth[0]);
     dfree(c, r.depth[1]);
+    dfree(c, r.depth4);
     dfree(c, r.hyp_photo);
     c->images.erase(it);
     return VSM_OK;
@@ -853,6 +863,7 @@ int vsm_set_depth_map(vsm_ctx *c, int32_t id, int32_t kind, const float *depth, 
     }
     HIPCHK(c, hipMemcpyAsync(r.depth[kind], depth, (size_t)w * h * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (kind == 0) ++r.depth_ver;
     return VSM_OK;
 }
 
@@ -1235,11 +1246,14 @@ int vsm_median_blur3(vsm_ctx *c, const float *in, int32_t e_size, int32_t n_size
 // ---------------------------------------------------------------------------
 // Serving: continuous batching of photometric solves.
 // ---------------------------------------------------------------------------
+// Device bytes a request needs while active: state, outputs, and the image's
+// photometric depth map that keeps its result if the image has none yet.
 static size_t serve_footprint(vsm_ctx *c, const ServeReq &r)
 {
     const ImageRec &ref = c->images[r.ref];
     const size_t hw = (size_t)ref.W * ref.H;
-    return state_bytes(ref.W, ref.H, (int)r.srcs.size()) + hw * 20 + 4096;
+    return state_bytes(ref.W, ref.H, (int)r.srcs.size()) + hw * 20 + (ref.depth[0] ? 0 : hw * 4) +
+           4096;
 }
 
 int vsm_submit(vsm_ctx *c, const vsm_view_req *req, uint64_t *ticket)
@@ -1295,6 +1309,11 @@ static int serve_admit(vsm_ctx *c)
         }
         const size_t hw = (size_t)r.vs.desc.W * r.vs.desc.H;
         rc = dalloc(c, &r.out_depth, hw * 5);
+        ImageRec &img = c->images[r.ref];
+        if (!rc && !img.depth[0]) {
+            rc = dalloc(c, &img.depth[0], hw);     // where the result is kept
+            if (rc) { dfree(c, r.out_depth); r.out_depth = nullptr; }
+        }
         if (rc) {
             release_view(c, r.vs);
             c->queue.push_front(std::move(owned));
@@ -1377,7 +1396,16 @@ int vsm_serve_step(vsm_ctx *c, int32_t *n_active, int32_t *n_queued)
                 HIPCHK(c, hipStreamSynchronize(c->stream));
                 dfree(c, dt);
             }
-            HIPCHK(c, launch_finalize(dv, 0, (int)hw, r.out_depth, r.out_normal, r.out_cost, nullptr, c->stream));
+            // the result also becomes the image's photometric depth map on the
+            // device (for fusion and later geometric passes), as with vsm_solve
+            auto im = c->images.find(r.ref);
+            float *keep = nullptr;
+            if (im != c->images.end() && im->second.depth[0]) {   // allocated at admission
+                keep = im->second.depth[0];
+                ++im->second.depth_ver;
+            }
+            HIPCHK(c, launch_finalize(dv, 0, (int)hw, r.out_depth, r.out_normal, r.out_cost, keep,
+                                      c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
             release_view(c, r.vs);
             r.done = true;

@@ -50,7 +50,7 @@ struct ViewDesc {
     float N_out[9];                 // (col,row,d)-frame normal -> camera frame
     float N_in[9];                  // camera frame -> (col,row,d) frame
     const uint32_t *src_quad[kMaxSrc];
-    const float *src_depth[kMaxSrc];
+    const float4 *src_depth4[kMaxSrc];   // photometric depth of each source, 2x2 per texel
     int src_w[kMaxSrc], src_h[kMaxSrc];
     float ptri[kMaxSrc];            // triangulation prior
     float tri[kMaxSrc];             // triangulation angle (rad)

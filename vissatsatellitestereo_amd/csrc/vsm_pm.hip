@@ -32,7 +32,7 @@ struct SrcLds {
     int base;                    // ay * W + ax: texel offset of the anchor
     float lox, hix, loy, hiy;    // in-image box in anchor-relative coordinates
     const uint32_t *quad;
-    const float *depth;
+    const float4 *depth4;        // photometric depth of the source, 2x2 per texel
     float ptri, tri, g0, g1, g2;
 };
 
@@ -83,7 +83,7 @@ __device__ void setup_source(const ViewDesc &V, int s, int uc, int vc, SrcLds &o
     o.lox = (float)(-ax); o.hix = (float)(o.W - 1 - ax);
     o.loy = (float)(-ay); o.hiy = (float)(o.H - 1 - ay);
     o.quad = V.src_quad[s];
-    o.depth = V.src_depth[s];
+    o.depth4 = V.src_depth4[s];
     o.ptri = V.ptri[s]; o.tri = V.tri[s];
     o.g0 = V.g[s][0]; o.g1 = V.g[s][1]; o.g2 = V.g[s][2];
 }
@@ -315,17 +315,17 @@ __device__ __forceinline__ float ncc_cost(const PMConst &c, const SrcLds &S, con
 __device__ float geo_error(const PMConst &c, const SrcLds &S, const Proj &P, float du, float dv)
 {
     const float gmax = c.geom_max;
-    if (S.depth == nullptr) return gmax;
+    if (S.depth4 == nullptr) return gmax;
     if (!(P.q0z > 0.f)) return gmax;
     const float px = P.q0x / P.q0z, py = P.q0y / P.q0z;
     if (!(px >= S.lox && px <= S.hix && py >= S.loy && py <= S.hiy)) return gmax;
     const float fx0 = floorf(px), fy0 = floorf(py);
     const int x0 = S.ax + (int)fx0, y0 = S.ay + (int)fy0;
-    const int x1 = min(x0 + 1, S.W - 1), y1 = min(y0 + 1, S.H - 1);
     const float fx = px - fx0, fy = py - fy0;
-    const gfloat_t D = (gfloat_t)S.depth;
-    const float da = D[y0 * S.W + x0], db = D[y0 * S.W + x1];
-    const float dc = D[y1 * S.W + x0], de = D[y1 * S.W + x1];
+    // one 16-byte load: d(x0,y0), d(x1,y0), d(x0,y1), d(x1,y1) with the
+    // neighbours clamped to the image
+    const float4 q = S.depth4[y0 * S.W + x0];
+    const float da = q.x, db = q.y, dc = q.z, de = q.w;
     if (!(da > 0.f && db > 0.f && dc > 0.f && de > 0.f)) return gmax;
     const float top = fmaf(fx, db - da, da), bot = fmaf(fx, de - dc, dc);
     const float ds = fmaf(fy, bot - top, top);
@@ -898,6 +898,18 @@ __global__ void k_pack_quad(const uint8_t *__restrict__ gray, int W, int H,
     quad[y * W + x] = a | (b << 8) | (c << 16) | (d << 24);
 }
 
+// Depth texture for the geometric term: texel (x,y) holds d(x,y), d(x1,y),
+// d(x,y1), d(x1,y1) with x1 = min(x+1, W-1), y1 = min(y+1, H-1).
+__global__ void k_pack_depth4(const float *__restrict__ d, int W, int H, float4 *__restrict__ out)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= W || y >= H) return;
+    const int x1 = min(x + 1, W - 1), y1 = min(y + 1, H - 1);
+    out[(size_t)y * W + x] = make_float4(d[(size_t)y * W + x], d[(size_t)y * W + x1],
+                                         d[(size_t)y1 * W + x], d[(size_t)y1 * W + x1]);
+}
+
 __global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -982,6 +994,13 @@ hipError_t launch_load_hyp(const ViewDesc *views, int view, int npix, const floa
 hipError_t launch_advance(ViewDesc *views, int nviews, hipStream_t st)
 {
     k_advance<<<(nviews + 63) / 64, 64, 0, st>>>(views, nviews);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_depth4(const float *depth, int W, int H, float4 *out, hipStream_t st)
+{
+    const dim3 grid((W + 255) / 256, H);
+    k_pack_depth4<<<grid, 256, 0, st>>>(depth, W, H, out);
     return hipGetLastError();
 }
 

@@ -32,10 +32,11 @@ def percentile(values, q):
 
 
 class ServeLoop:
-    def __init__(self, engine, clock=time.perf_counter, want=("depth",)):
+    def __init__(self, engine, clock=time.perf_counter, want=("depth",), on_complete=None):
         self.engine = engine
         self.clock = clock
         self.want = want
+        self.on_complete = on_complete   # f(ticket, view, result) when a request finishes
         self.pending = {}      # ticket -> (view, arrival time)
         self.done = {}         # ticket -> (result, arrival, completion)
 
@@ -54,6 +55,8 @@ class ServeLoop:
             if res is not None:
                 self.done[t] = (res, arr, now)
                 del self.pending[t]
+                if self.on_complete is not None:
+                    self.on_complete(t, view, res)
         return n_active, n_queued
 
     def drain(self, max_steps=10**7):

@@ -188,6 +188,13 @@ int vsm_remove_image(vsm_ctx *ctx, int32_t image_id);
 int vsm_set_depth_map(vsm_ctx *ctx, int32_t image_id, int32_t kind, const float *depth, int32_t w,
                       int32_t h);
 
+/* Provide the normal map that goes with a depth map (kind 0 or 1): w*h*3
+ * floats, unit normals in the camera frame facing the camera, as in
+ * normal_maps/<img>.<kind>.bin.  vsm_solve keeps its own normals the same
+ * way; vsm_fuse_points reads them. */
+int vsm_set_normal_map(vsm_ctx *ctx, int32_t image_id, int32_t kind, const float *normal,
+                       int32_t w, int32_t h);
+
 /* Copy the depth map of kind 0 (photometric) or 1 (geometric) the context
  * holds for image_id into a caller-owned w*h float buffer: the map a solve
  * left on the device, i.e. what depth_maps/<img>.<kind>.bin holds after the
@@ -262,6 +269,42 @@ int vsm_fuse_dsm(vsm_ctx *ctx, const vsm_dsm_grid *grid, const int32_t *image_id
  * e_size FP64 grid with per-cell nanmax and the 3x3 hole fill. */
 int vsm_proj_to_grid(vsm_ctx *ctx, const double *points, int64_t n, const vsm_dsm_grid *grid,
                      double *dsm_out);
+
+/* 3D fusion of depth maps into a point cloud, the path of aggregate_3d.py
+ * (aggregate_3d.py:44-82: `colmap stereo_fusion --input_type geometric
+ * --StereoFusion.min_num_pixels 4 --StereoFusion.max_reproj_error 2
+ * --StereoFusion.max_depth_error 1.0 --StereoFusion.max_normal_error 10`),
+ * order-independent: a pixel p of image i with depth d > 0 back-projects to
+ * X; image j is consistent with it when X projects inside j onto pixel q
+ * (nearest) with depth d_j > 0, the back-projection Y of q lies within
+ * max_depth_error metres of X, Y re-projects into i within
+ * max_reproj_error pixels of p, and the two world-frame normals differ by at
+ * most max_normal_error degrees.  p yields a point when at least
+ * min_num_pixels pixels (p included) agree and no consistent image has a
+ * smaller index than i (each cluster is emitted once, by its first image):
+ * the mean of X and the Y's, the normalised sum of the normals and the
+ * rounded mean grey value.  Points come out in image order, then row-major
+ * pixel order. */
+typedef struct {
+    int32_t min_num_pixels;           /* 4 */
+    float max_reproj_error;           /* 2 px */
+    float max_depth_error;            /* 1.0 m */
+    float max_normal_error_deg;       /* 10 degrees */
+} vsm_fuse3d_params;
+
+void vsm_default_fuse3d_params(vsm_fuse3d_params *p);
+
+/* image_ids: n images with depth (kind) and normal maps held by the context.
+ * inv_proj_mats: optional n*16 inverse projections (NULL: the context's);
+ * cam_to_world: n*9 row-major rotations taking camera-frame normals to the
+ * world (ENU) frame (R^T of P4 = [K R | K t; ...]).  Writes up to capacity
+ * points (xyz n x 3 FP64, normal n x 3 float or NULL, grey n uint8 or NULL)
+ * and sets *count to the total, which may exceed capacity (nothing beyond
+ * capacity is written; call again with a larger buffer). */
+int vsm_fuse_points(vsm_ctx *ctx, const int32_t *image_ids, int32_t n, int32_t depth_kind,
+                    const vsm_fuse3d_params *params, const double *inv_proj_mats,
+                    const double *cam_to_world, int64_t capacity, double *xyz, float *normal,
+                    uint8_t *grey, int64_t *count);
 
 /* 3x3 median with replicated border on an n_size x e_size float raster
  * (cv2.medianBlur(dsm, 3) at produce_dsm.py:58 and aggregate_2p5d.py:81);

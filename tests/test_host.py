@@ -153,7 +153,8 @@ def test_struct_layouts_match_header(tmp_path):
     from vissatsatellitestereo_amd import _lib
     checks = [("vsm_params", _lib.Params), ("vsm_view_req", _lib.ViewReq),
               ("vsm_view_out", _lib.ViewOut), ("vsm_dsm_grid", _lib.DsmGrid), ("vsm_stats", _lib.Stats),
-              ("vsm_geo_params", _lib.GeoParams), ("vsm_geo_frame", _lib.GeoFrame)]
+              ("vsm_geo_params", _lib.GeoParams), ("vsm_geo_frame", _lib.GeoFrame),
+              ("vsm_fuse3d_params", _lib.Fuse3dParams)]
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "vissat_mvs.h"', "int main(void){"]
     for cname, py in checks:
         lines.append(f'printf("%zu\\n", sizeof({cname}));')

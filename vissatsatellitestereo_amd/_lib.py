@@ -64,6 +64,11 @@ class DsmGrid(ctypes.Structure):
                 ("n_res", ctypes.c_double), ("e_size", ctypes.c_int32), ("n_size", ctypes.c_int32)]
 
 
+class Fuse3dParams(ctypes.Structure):
+    _fields_ = [("min_num_pixels", ctypes.c_int32), ("max_reproj_error", ctypes.c_float),
+                ("max_depth_error", ctypes.c_float), ("max_normal_error_deg", ctypes.c_float)]
+
+
 class GeoParams(ctypes.Structure):
     _fields_ = [("lat0_deg", ctypes.c_double), ("lon0_deg", ctypes.c_double), ("alt0", ctypes.c_double),
                 ("zone_number", ctypes.c_int32), ("south", ctypes.c_int32)]
@@ -130,6 +135,12 @@ SIGNATURES = [
                                     P(_c.c_double)]),
     ("vsm_median_blur3", _c.c_int, [_c.c_void_p, P(_c.c_float), _c.c_int32, _c.c_int32,
                                     P(_c.c_float)]),
+    ("vsm_set_normal_map", _c.c_int, [_c.c_void_p, _c.c_int32, _c.c_int32, P(_c.c_float), _c.c_int32,
+                                      _c.c_int32]),
+    ("vsm_default_fuse3d_params", None, [P(Fuse3dParams)]),
+    ("vsm_fuse_points", _c.c_int, [_c.c_void_p, P(_c.c_int32), _c.c_int32, _c.c_int32, P(Fuse3dParams),
+                                   P(_c.c_double), P(_c.c_double), _c.c_int64, P(_c.c_double),
+                                   P(_c.c_float), P(_c.c_uint8), P(_c.c_int64)]),
     ("vsm_geo_frame_init", _c.c_int, [P(GeoParams), P(GeoFrame)]),
     ("vsm_set_geo_frame", _c.c_int, [_c.c_void_p, P(GeoFrame)]),
     ("vsm_enu_to_utm", _c.c_int, [_c.c_void_p, P(_c.c_double), _c.c_int64, P(_c.c_double)]),

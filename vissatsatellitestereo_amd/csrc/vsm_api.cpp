@@ -36,6 +36,7 @@ struct ImageRec {
     double P4[16];
     double Pinv[16];
     float *depth[2] = {nullptr, nullptr};   // photometric / geometric depth kept on device
+    float *normal[2] = {nullptr, nullptr};  // camera-frame normals that go with depth[]
     float4 *depth4 = nullptr;               // photometric depth, 2x2 neighbourhood per texel
     uint64_t depth_ver = 0, depth4_ver = 0; // depth4 is current when the versions agree
     float4 *hyp_photo = nullptr;            // photometric hypotheses (geometric init)
@@ -673,18 +674,25 @@ int finalize_outputs(vsm_ctx *c, ViewState &vs, int pass, int ref_id, vsm_view_o
     ViewDesc *dv = nullptr;
     float *tmp = nullptr;
     int rc = dalloc(c, &dv, 1);
-    if (!rc) rc = dalloc(c, &tmp, hw * 5);
+    if (!rc) rc = dalloc(c, &tmp, hw * 2);
     if (rc) { dfree(c, dv); return rc; }
-    if (!ref.depth[pass] && (rc = dalloc(c, &ref.depth[pass], hw))) { dfree(c, dv); dfree(c, tmp); return rc; }
+    if ((!ref.depth[pass] && (rc = dalloc(c, &ref.depth[pass], hw))) ||
+        (!ref.normal[pass] && (rc = dalloc(c, &ref.normal[pass], hw * 3)))) {
+        dfree(c, dv);
+        dfree(c, tmp);
+        return rc;
+    }
     HIPCHK(c, hipMemcpyAsync(dv, &vs.desc, sizeof(ViewDesc), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, launch_finalize(dv, 0, (int)hw, tmp, tmp + hw, tmp + 4 * hw, ref.depth[pass], c->stream));
+    HIPCHK(c, launch_finalize(dv, 0, (int)hw, tmp, ref.normal[pass], tmp + hw, ref.depth[pass],
+                              c->stream));
     if (pass == 0) ++ref.depth_ver;
     if (out && out->depth)
         HIPCHK(c, hipMemcpyAsync(out->depth, tmp, hw * 4, hipMemcpyDeviceToHost, c->stream));
     if (out && out->normal)
-        HIPCHK(c, hipMemcpyAsync(out->normal, tmp + hw, hw * 12, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(out->normal, ref.normal[pass], hw * 12, hipMemcpyDeviceToHost,
+                                 c->stream));
     if (out && out->cost)
-        HIPCHK(c, hipMemcpyAsync(out->cost, tmp + 4 * hw, hw * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(out->cost, tmp + hw, hw * 4, hipMemcpyDeviceToHost, c->stream));
     if (out && out->consistency) {
         if (d_mask) HIPCHK(c, hipMemcpyAsync(out->consistency, d_mask, hw * 4, hipMemcpyDeviceToHost, c->stream));
         else memset(out->consistency, 0, hw * 4);
@@ -843,6 +851,8 @@ int vsm_remove_image(vsm_ctx *c, int32_t id)
     dfree(c, r.quad);
     dfree(c, r.depth[0]);
     dfree(c, r.depth[1]);
+    dfree(c, r.normal[0]);
+    dfree(c, r.normal[1]);
     dfree(c, r.depth4);
     dfree(c, r.hyp_photo);
     c->images.erase(it);
@@ -864,6 +874,25 @@ int vsm_set_depth_map(vsm_ctx *c, int32_t id, int32_t kind, const float *depth, 
     HIPCHK(c, hipMemcpyAsync(r.depth[kind], depth, (size_t)w * h * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (kind == 0) ++r.depth_ver;
+    return VSM_OK;
+}
+
+int vsm_set_normal_map(vsm_ctx *c, int32_t id, int32_t kind, const float *normal, int32_t w,
+                       int32_t h)
+{
+    if (!c || !normal) return VSM_EINVAL;
+    if (kind != 0 && kind != 1) return fail(c, VSM_EINVAL, "normal map kind must be 0 or 1");
+    auto it = c->images.find(id);
+    if (it == c->images.end()) return fail(c, VSM_ENOENT, "unknown image %d", id);
+    ImageRec &r = it->second;
+    if (w != r.W || h != r.H) return fail(c, VSM_EINVAL, "normal map %dx%d != image %dx%d", w, h, r.W, r.H);
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!r.normal[kind]) {
+        int rc = dalloc(c, &r.normal[kind], (size_t)w * h * 3);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipMemcpyAsync(r.normal[kind], normal, (size_t)w * h * 12, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return VSM_OK;
 }
 
@@ -1106,6 +1135,114 @@ int vsm_fuse_dsm(vsm_ctx *c, const vsm_dsm_grid *grid, const int32_t *ids, int32
         HIPCHK(c, hipMemcpyAsync(per_view_out, stack, ncell * 4 * n, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     dfree(c, cells); dfree(c, stack); dfree(c, tmp); dfree(c, fused); dfree(c, mats);
+    return VSM_OK;
+}
+
+void vsm_default_fuse3d_params(vsm_fuse3d_params *p)
+{
+    if (!p) return;
+    p->min_num_pixels = 4;           // aggregate_3d.py:47-50
+    p->max_reproj_error = 2.f;
+    p->max_depth_error = 1.f;
+    p->max_normal_error_deg = 10.f;
+}
+
+int vsm_fuse_points(vsm_ctx *c, const int32_t *ids, int32_t n, int32_t depth_kind,
+                    const vsm_fuse3d_params *params, const double *inv_proj_mats,
+                    const double *cam_to_world, int64_t capacity, double *xyz, float *normal,
+                    uint8_t *grey, int64_t *count)
+{
+    if (!c) return VSM_EINVAL;
+    if (!ids || n <= 0 || n > 64) return fail(c, VSM_EINVAL, "1..64 images required, got %d", n);
+    if (depth_kind != 0 && depth_kind != 1) return fail(c, VSM_EINVAL, "depth_kind must be 0 or 1");
+    if (!params || !cam_to_world || !count || capacity < 0 || (capacity > 0 && !xyz))
+        return fail(c, VSM_EINVAL, "bad params / cam_to_world / outputs");
+    if (params->min_num_pixels < 1 || !(params->max_reproj_error >= 0.f) ||
+        !(params->max_depth_error >= 0.f) || !(params->max_normal_error_deg >= 0.f) ||
+        !(params->max_normal_error_deg <= 180.f))
+        return fail(c, VSM_EINVAL, "invalid fusion thresholds");
+    std::vector<Fuse3dView> hv(n);
+    long long max_hw = 0;
+    for (int v = 0; v < n; ++v) {
+        auto it = c->images.find(ids[v]);
+        if (it == c->images.end()) return fail(c, VSM_ENOENT, "unknown image %d", ids[v]);
+        const ImageRec &r = it->second;
+        if (!r.depth[depth_kind] || !r.normal[depth_kind])
+            return fail(c, VSM_EINVAL, "image %d has no %s depth + normal maps", ids[v],
+                        depth_kind ? "geometric" : "photometric");
+        Fuse3dView &f = hv[v];
+        f.depth = r.depth[depth_kind];
+        f.normal = r.normal[depth_kind];
+        f.gray = r.gray;
+        f.W = r.W;
+        f.H = r.H;
+        memcpy(f.P, r.P4, sizeof f.P);
+        memcpy(f.Pinv, inv_proj_mats ? inv_proj_mats + 16 * v : r.Pinv, sizeof f.Pinv);
+        for (int k = 0; k < 9; ++k) f.R[k] = (float)cam_to_world[9 * v + k];
+        max_hw = std::max(max_hw, (long long)r.W * r.H);
+    }
+    Fuse3dConst k;
+    k.n = n;
+    k.min_num = params->min_num_pixels;
+    k.max_depth_err = (double)params->max_depth_error;
+    k.max_reproj = (double)params->max_reproj_error;
+    k.cos_max_normal = (float)std::cos((double)params->max_normal_error_deg * M_PI / 180.0);
+    HIPCHK(c, hipSetDevice(c->device));
+    Fuse3dView *dviews = nullptr;
+    FusedPoint *dense = nullptr, *sel = nullptr;
+    int *flag = nullptr, *nsel = nullptr;
+    double *dxyz = nullptr;
+    float *dnrm = nullptr;
+    uint8_t *dgray = nullptr, *temp = nullptr;
+    size_t temp_bytes = 0;
+    HIPCHK(c, compact_fused(nullptr, nullptr, max_hw, nullptr, temp_bytes, nullptr, nullptr, c->stream));
+    int rc = dalloc(c, &dviews, n);
+    if (!rc) rc = dalloc(c, &dense, max_hw);
+    if (!rc) rc = dalloc(c, &sel, max_hw);
+    if (!rc) rc = dalloc(c, &flag, max_hw);
+    if (!rc) rc = dalloc(c, &nsel, 1);
+    if (!rc) rc = dalloc(c, &dxyz, 3 * (size_t)max_hw);
+    if (!rc) rc = dalloc(c, &dnrm, 3 * (size_t)max_hw);
+    if (!rc) rc = dalloc(c, &dgray, max_hw);
+    if (!rc) rc = dalloc(c, &temp, std::max<size_t>(temp_bytes, 1));
+    auto cleanup = [&]() {
+        dfree(c, dviews); dfree(c, dense); dfree(c, sel); dfree(c, flag); dfree(c, nsel);
+        dfree(c, dxyz); dfree(c, dnrm); dfree(c, dgray); dfree(c, temp);
+    };
+    if (rc) { cleanup(); return rc; }
+#define F3CHK(expr)                                                                             \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            cleanup();                                                                          \
+            return fail(c, VSM_EDEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));        \
+        }                                                                                       \
+    } while (0)
+    F3CHK(hipMemcpyAsync(dviews, hv.data(), n * sizeof(Fuse3dView), hipMemcpyHostToDevice, c->stream));
+    int64_t total = 0;
+    for (int v = 0; v < n; ++v) {
+        const long long hw = (long long)hv[v].W * hv[v].H;
+        F3CHK(launch_fuse3d(dviews, v, hv[v].W, hv[v].H, k, dense, flag, c->stream));
+        size_t tb = temp_bytes;
+        F3CHK(compact_fused(dense, flag, hw, temp, tb, sel, nsel, c->stream));
+        int ns = 0;
+        F3CHK(hipMemcpyAsync(&ns, nsel, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        F3CHK(hipStreamSynchronize(c->stream));
+        const int64_t room = std::max<int64_t>(0, capacity - total);
+        const int64_t put = std::min<int64_t>(room, ns);
+        if (put > 0) {
+            F3CHK(launch_unpack_fused(sel, put, dxyz, dnrm, dgray, c->stream));
+            F3CHK(hipMemcpyAsync(xyz + 3 * total, dxyz, put * 24, hipMemcpyDeviceToHost, c->stream));
+            if (normal)
+                F3CHK(hipMemcpyAsync(normal + 3 * total, dnrm, put * 12, hipMemcpyDeviceToHost, c->stream));
+            if (grey) F3CHK(hipMemcpyAsync(grey + total, dgray, put, hipMemcpyDeviceToHost, c->stream));
+            F3CHK(hipStreamSynchronize(c->stream));
+        }
+        total += ns;
+    }
+#undef F3CHK
+    cleanup();
+    *count = total;
     return VSM_OK;
 }
 

@@ -21,6 +21,8 @@
 #include "vsm_geo.h"
 #include "vsm_kernels.h"
 
+#include <hipcub/hipcub.hpp>
+
 namespace vsm {
 
 __device__ __forceinline__ unsigned long long enc_f64(double v)
@@ -103,6 +105,120 @@ __global__ void k_enu_to_utm(const double *__restrict__ enu, long long n, vsm_ge
     out[3 * i] = ok ? E : nan;
     out[3 * i + 1] = ok ? N : nan;
     out[3 * i + 2] = ok ? alt : nan;
+}
+
+// ---------------------------------------------------------------------------
+// 3D fusion: one thread per pixel of image i (see vsm_fuse_points).  FP64
+// projections with a fixed operation order, FP32 normals.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void backproject4(const double *Mi, double col, double row, double d,
+                                             double &X, double &Y, double &Z)
+{
+    double h[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        h[k] = ((Mi[4 * k + 0] * col + Mi[4 * k + 1] * row) + Mi[4 * k + 2]) + Mi[4 * k + 3] * d;
+    X = h[0] / h[3];
+    Y = h[1] / h[3];
+    Z = h[2] / h[3];
+}
+
+__device__ __forceinline__ void project3(const double *P, double X, double Y, double Z, double &u0,
+                                         double &u1, double &u2)
+{
+    u0 = ((P[0] * X + P[1] * Y) + P[2] * Z) + P[3];
+    u1 = ((P[4] * X + P[5] * Y) + P[6] * Z) + P[7];
+    u2 = ((P[8] * X + P[9] * Y) + P[10] * Z) + P[11];
+}
+
+__device__ __forceinline__ void rotate_normal(const float *R, const float *n, float &a, float &b,
+                                              float &c)
+{
+    a = (R[0] * n[0] + R[1] * n[1]) + R[2] * n[2];
+    b = (R[3] * n[0] + R[4] * n[1]) + R[5] * n[2];
+    c = (R[6] * n[0] + R[7] * n[1]) + R[8] * n[2];
+}
+
+__global__ void k_fuse3d(const Fuse3dView *__restrict__ V, int i, Fuse3dConst k,
+                         FusedPoint *__restrict__ out, int *__restrict__ flag)
+{
+    const Fuse3dView &A = V[i];
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long long)A.W * A.H) return;
+    flag[p] = 0;
+    const float d = A.depth[p];
+    if (!(d > 0.f) || !isfinite(d)) return;
+    const double col = (double)(p % A.W), row = (double)(p / A.W);
+    double X, Y, Z;
+    backproject4(A.Pinv, col, row, (double)d, X, Y, Z);
+    if (!isfinite(X) || !isfinite(Y) || !isfinite(Z)) return;
+    float na, nb, nc;
+    rotate_normal(A.R, A.normal + 3 * p, na, nb, nc);
+    int count = 1;
+    double sx = X, sy = Y, sz = Z;
+    float snx = na, sny = nb, snz = nc;
+    uint32_t sg = A.gray ? A.gray[p] : 0u;
+    for (int j = 0; j < k.n; ++j) {
+        if (j == i) continue;
+        const Fuse3dView &B = V[j];
+        double u0, u1, u2;
+        project3(B.P, X, Y, Z, u0, u1, u2);
+        if (!(u2 > 0.0)) continue;
+        const double cx = floor(u0 / u2 + 0.5), cy = floor(u1 / u2 + 0.5);
+        if (!(cx >= 0.0 && cy >= 0.0 && cx < (double)B.W && cy < (double)B.H)) continue;
+        const long long q = (long long)cy * B.W + (long long)cx;
+        const float dj = B.depth[q];
+        if (!(dj > 0.f) || !isfinite(dj)) continue;
+        double Xj, Yj, Zj;
+        backproject4(B.Pinv, cx, cy, (double)dj, Xj, Yj, Zj);
+        const double ex = Xj - X, ey = Yj - Y, ez = Zj - Z;
+        if (!(sqrt((ex * ex + ey * ey) + ez * ez) <= k.max_depth_err)) continue;
+        double v0, v1, v2;
+        project3(A.P, Xj, Yj, Zj, v0, v1, v2);
+        if (!(v2 > 0.0)) continue;
+        const double rx = v0 / v2 - col, ry = v1 / v2 - row;
+        if (!(sqrt(rx * rx + ry * ry) <= k.max_reproj)) continue;
+        float ma, mb, mc;
+        rotate_normal(B.R, B.normal + 3 * q, ma, mb, mc);
+        if (!((na * ma + nb * mb) + nc * mc >= k.cos_max_normal)) continue;
+        if (j < i) return;                 // the cluster belongs to image j
+        ++count;
+        sx = sx + Xj;
+        sy = sy + Yj;
+        sz = sz + Zj;
+        snx = snx + ma;
+        sny = sny + mb;
+        snz = snz + mc;
+        sg += B.gray ? B.gray[q] : 0u;
+    }
+    if (count < k.min_num) return;
+    const double fc = (double)count;
+    const float l = sqrtf((snx * snx + sny * sny) + snz * snz);
+    FusedPoint o;
+    o.x = sx / fc;
+    o.y = sy / fc;
+    o.z = sz / fc;
+    o.nx = l > 0.f ? snx / l : 0.f;
+    o.ny = l > 0.f ? sny / l : 0.f;
+    o.nz = l > 0.f ? snz / l : 0.f;
+    o.gray = (sg + (uint32_t)(count / 2)) / (uint32_t)count;
+    out[p] = o;
+    flag[p] = 1;
+}
+
+__global__ void k_unpack_fused(const FusedPoint *__restrict__ s, long long n, double *__restrict__ xyz,
+                               float *__restrict__ normal, uint8_t *__restrict__ gray)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const FusedPoint o = s[t];
+    xyz[3 * t] = o.x;
+    xyz[3 * t + 1] = o.y;
+    xyz[3 * t + 2] = o.z;
+    normal[3 * t] = o.nx;
+    normal[3 * t + 1] = o.ny;
+    normal[3 * t + 2] = o.nz;
+    gray[t] = (uint8_t)min(o.gray, 255u);
 }
 
 // Small in-register sort (insertion) of up to 9 doubles/floats.
@@ -264,6 +380,29 @@ hipError_t launch_rasterize_view(const float *depth, int W, int H, const double 
     else
         k_rasterize_view<false><<<nblk((long long)W * H, 256), 256, 0, st>>>(
             depth, W, H, invP4_dev, g, vsm_geo_frame{}, cell_max);
+    return hipGetLastError();
+}
+
+hipError_t launch_fuse3d(const Fuse3dView *views, int i, int W, int H, Fuse3dConst k, FusedPoint *out,
+                         int *flag, hipStream_t st)
+{
+    const long long n = (long long)W * H;
+    if (n <= 0) return hipSuccess;
+    k_fuse3d<<<nblk(n, 256), 256, 0, st>>>(views, i, k, out, flag);
+    return hipGetLastError();
+}
+
+hipError_t compact_fused(const FusedPoint *pts, const int *flag, long long n, void *temp,
+                         size_t &temp_bytes, FusedPoint *sel, int *n_sel, hipStream_t st)
+{
+    return hipcub::DeviceSelect::Flagged(temp, temp_bytes, pts, flag, sel, n_sel, (int)n, st);
+}
+
+hipError_t launch_unpack_fused(const FusedPoint *sel, long long n, double *xyz, float *normal,
+                               uint8_t *gray, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    k_unpack_fused<<<nblk(n, 256), 256, 0, st>>>(sel, n, xyz, normal, gray);
     return hipGetLastError();
 }
 

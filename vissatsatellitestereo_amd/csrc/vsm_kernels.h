@@ -30,6 +30,35 @@ struct DsmGridDev {
     double ul_e, ul_n, e_res, n_res;
     int e_size, n_size;
 };
+// 3D fusion (vsm_fuse.hip)
+struct Fuse3dView {
+    const float *depth;             // W*H
+    const float *normal;            // W*H*3, camera frame
+    const uint8_t *gray;            // W*H
+    int W, H;
+    double P[16];                   // P4 (projection, rows 0..2 used)
+    double Pinv[16];                // back-projection
+    float R[9];                     // camera -> world rotation of normals
+};
+struct Fuse3dConst {
+    int n, min_num;
+    double max_depth_err, max_reproj;
+    float cos_max_normal;
+};
+struct FusedPoint {
+    double x, y, z;
+    float nx, ny, nz;
+    uint32_t gray;
+};
+hipError_t launch_fuse3d(const Fuse3dView *views, int i, int W, int H, Fuse3dConst k, FusedPoint *out,
+                         int *flag, hipStream_t st);
+// Compact the flagged points of one image into xyz / normal / gray at their
+// running offsets; *n_sel receives the number selected (device int).
+hipError_t compact_fused(const FusedPoint *pts, const int *flag, long long n, void *temp,
+                         size_t &temp_bytes, FusedPoint *sel, int *n_sel, hipStream_t st);
+hipError_t launch_unpack_fused(const FusedPoint *sel, long long n, double *xyz, float *normal,
+                               uint8_t *gray, hipStream_t st);
+
 // geo: NULL = grid in the ENU frame of the points; else the UTM grid.
 hipError_t launch_rasterize_view(const float *depth, int W, int H, const double *invP4_dev,
                                  DsmGridDev g, const vsm_geo_frame *geo,

@@ -90,6 +90,41 @@ class MvsContext:
         self._check(self.lib.vsm_remove_image(self.h, int(image_id)))
         self.sizes.pop(int(image_id), None)
 
+    def set_normal_map(self, image_id, normal, kind=1):
+        """Camera-frame unit normals (H x W x 3) that go with the depth map."""
+        n = np.ascontiguousarray(normal, dtype=np.float32)
+        h, w = n.shape[:2]
+        self._check(self.lib.vsm_set_normal_map(self.h, int(image_id), int(kind), fptr(n), w, h))
+
+    def fuse_points(self, image_ids, cam_to_world, depth_kind=1, inv_proj_mats=None, capacity=None,
+                    **thresholds):
+        """3D fusion (vsm_fuse_points).  Returns (xyz N x 3 FP64, normals N x 3
+        float32, grey N uint8)."""
+        p = _lib.Fuse3dParams()
+        self.lib.vsm_default_fuse3d_params(ctypes.byref(p))
+        for k, v in thresholds.items():
+            setattr(p, k, v)
+        ids = np.ascontiguousarray(image_ids, dtype=np.int32)
+        R = np.ascontiguousarray(cam_to_world, dtype=np.float64).reshape(len(ids), 9)
+        mats = None
+        if inv_proj_mats is not None:
+            mats = np.ascontiguousarray(inv_proj_mats, dtype=np.float64).reshape(len(ids), 16)
+        dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if a is not None else None
+        cap = int(capacity) if capacity is not None else sum(
+            self.sizes[int(i)][0] * self.sizes[int(i)][1] for i in ids) // 2 + 1
+        while True:
+            xyz = np.zeros((cap, 3), np.float64)
+            nrm = np.zeros((cap, 3), np.float32)
+            gry = np.zeros(cap, np.uint8)
+            cnt = ctypes.c_int64()
+            self._check(self.lib.vsm_fuse_points(
+                self.h, ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(ids), int(depth_kind),
+                ctypes.byref(p), dp(mats), dp(R), cap, dp(xyz), fptr(nrm),
+                gry.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), ctypes.byref(cnt)))
+            if cnt.value <= cap:
+                return xyz[:cnt.value], nrm[:cnt.value], gry[:cnt.value]
+            cap = cnt.value
+
     def get_depth_map(self, image_id, kind=1):
         """The depth map (kind 0 photometric, 1 geometric) held on the device."""
         h, w = self.sizes[int(image_id)]

@@ -24,6 +24,8 @@
 namespace vsm {
 
 constexpr int kGroupA = 4;      // candidate hypotheses per shared window pass (4 + 3)
+constexpr int kFarChunk = 8;    // propagation candidates loaded together per direction
+constexpr int kProbChunk = 4;   // sources whose neighbour probabilities are loaded together
 
 struct SrcLds {
     float A[12];                 // tile-centred ref -> src (anchor-relative), normalised
@@ -148,6 +150,14 @@ __device__ __forceinline__ float sample_quad(const SrcView &S, float px, float p
     return fmaf(fy, bot - top, top);
 }
 
+// Image value of the texel containing anchor-relative (px, py) (inside the
+// image): the offset the source samples of a window are centred on.
+__device__ __forceinline__ float texel_at(const SrcView &S, float px, float py)
+{
+    const int idx = __mul24(cvt_floor(py), S.W) + (cvt_floor(px) + S.base);
+    return (float)(((gquad_t)S.quad)[(unsigned)idx] & 0xFFu);
+}
+
 // Bilateral weight of a tap: 2^-(r^2 ccc + (dx^2+dy^2) css), the hardware
 // exp2 (weights below 2^-126 flush to zero, which no sum can notice).
 __device__ __forceinline__ float tap_weight(const PMConst &c, float r, int dx, int dy)
@@ -250,7 +260,7 @@ __device__ __forceinline__ void window_pass(const PMConst &c, const SrcView (&SV
 }
 
 // NCC cost of G plane hypotheses.  Each hypothesis' source samples are
-// centred on its own sample at the window centre, the reference samples on
+// centred on the source texel at its window centre, the reference samples on
 // the reference centre value (NCC is invariant to the offsets; they keep the
 // FP32 moment sums well conditioned when the two views differ in brightness).
 // SS[k] is the source of hypothesis k: all the same source (several planes
@@ -273,7 +283,7 @@ __device__ __forceinline__ void ncc_cost_group(const PMConst &c, const SrcLds *c
         iz0[k] = valid[k] ? __builtin_amdgcn_rcpf(P[k].q0z) : 1.f;
         const float cx = P[k].q0x * iz0[k], cy = P[k].q0y * iz0[k];
         valid[k] = valid[k] && cx >= S.lox && cx <= S.hix && cy >= S.loy && cy <= S.hiy;
-        sc[k] = valid[k] ? sample_quad(S, cx, cy) : 0.f;
+        sc[k] = valid[k] ? texel_at(S, cx, cy) : 0.f;
         const float delta = fr * (fabsf(P[k].huz) + fabsf(P[k].hvz)) * fabsf(iz0[k]);
         far = far || !(delta <= kDeltaMax);
         // conservative half-extent of the projected window (+1 px guard)
@@ -317,7 +327,8 @@ __device__ float geo_error(const PMConst &c, const SrcLds &S, const Proj &P, flo
     const float gmax = c.geom_max;
     if (S.depth4 == nullptr) return gmax;
     if (!(P.q0z > 0.f)) return gmax;
-    const float px = P.q0x / P.q0z, py = P.q0y / P.q0z;
+    const float iq = __builtin_amdgcn_rcpf(P.q0z);
+    const float px = P.q0x * iq, py = P.q0y * iq;
     if (!(px >= S.lox && px <= S.hix && py >= S.loy && py <= S.hiy)) return gmax;
     const float fx0 = floorf(px), fy0 = floorf(py);
     const int x0 = S.ax + (int)fx0, y0 = S.ay + (int)fy0;
@@ -333,7 +344,8 @@ __device__ float geo_error(const PMConst &c, const SrcLds &S, const Proj &P, flo
     const float by = fmaf(S.B[4], px, fmaf(S.B[5], py, fmaf(S.B[7], ds, S.B[6])));
     const float bz = fmaf(S.B[8], px, fmaf(S.B[9], py, fmaf(S.B[11], ds, S.B[10])));
     if (!(bz > 0.f)) return gmax;
-    const float ex = bx / bz - du, ey = by / bz - dv;
+    const float ib = __builtin_amdgcn_rcpf(bz);
+    const float ex = bx * ib - du, ey = by * ib - dv;
     return sqrtf(ex * ex + ey * ey);
 }
 
@@ -431,7 +443,7 @@ __device__ __forceinline__ void tile_prologue(const ViewDesc &V, const Tile &t, 
 template <int MAXS>
 __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, const PMConst &c,
                                               float *s_ref, SrcLds *s_src, float *s_c0,
-                                              uint16_t *s_list, unsigned int &my_ncc,
+                                              uint32_t *s_list, unsigned int &my_ncc,
                                               unsigned int &my_geo)
 {
     const int iter = V.step >> 1, color = V.step & 1, pass = V.pass;
@@ -454,16 +466,29 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
         const float4 h0 = V.hyp[p];
         hd[0] = h0.x; hn0[0] = h0.y; hn1[0] = h0.z; hn2[0] = h0.w;
         const int DX[4] = {-1, 1, 0, 0}, DY[4] = {0, 0, -1, 1};
+        // lowest-cost neighbour at odd distances 1, 3, .., 2 n_far - 1 along
+        // each direction (the first of equal costs); the candidate costs of a
+        // chunk are loaded together, then compared in order
 #pragma unroll
         for (int dir = 0; dir < 4; ++dir) {
             int bx = -1, by = -1;
             float bc = 0.f;
-            for (int k = 0; k < c.n_far; ++k) {
-                const int dist = 2 * k + 1;
-                const int qx = x + DX[dir] * dist, qy = y + DY[dir] * dist;
-                if (qx < 0 || qy < 0 || qx >= W || qy >= H) break;
-                const float cq = V.cost[qy * W + qx];
-                if (bx < 0 || cq < bc) { bx = qx; by = qy; bc = cq; }
+            for (int k0 = 0; k0 < c.n_far; k0 += kFarChunk) {
+                float cq[kFarChunk];
+#pragma unroll
+                for (int kk = 0; kk < kFarChunk; ++kk) {
+                    const int k = k0 + kk, dist = 2 * k + 1;
+                    const int qx = x + DX[dir] * dist, qy = y + DY[dir] * dist;
+                    const bool ok = k < c.n_far && qx >= 0 && qy >= 0 && qx < W && qy < H;
+                    cq[kk] = ok ? V.cost[qy * W + qx] : 0.f;
+                }
+#pragma unroll
+                for (int kk = 0; kk < kFarChunk; ++kk) {
+                    const int k = k0 + kk, dist = 2 * k + 1;
+                    const int qx = x + DX[dir] * dist, qy = y + DY[dir] * dist;
+                    const bool ok = k < c.n_far && qx >= 0 && qy >= 0 && qx < W && qy < H;
+                    if (ok && (bx < 0 || cq[kk] < bc)) { bx = qx; by = qy; bc = cq[kk]; }
+                }
             }
             if (bx < 0) {
                 hd[1 + dir] = hd[0]; hn0[1 + dir] = hn0[0]; hn1[1 + dir] = hn1[0]; hn2[1 + dir] = hn2[0];
@@ -537,16 +562,35 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
         }
         float tot = 0.f;
         int lastpos = 0;
-        for (int s = 0; s < S; ++s) {
+        bool inb[4];
+#pragma unroll
+        for (int dir = 0; dir < 4; ++dir) {
+            const int qx = x + DX[dir], qy = y + DY[dir];
+            inb[dir] = qx >= 0 && qy >= 0 && qx < W && qy < H;
+        }
+        for (int s0 = 0; s0 < S; s0 += kProbChunk) {
+          // the neighbours' selection probabilities of a chunk of sources,
+          // loaded together
+          float pn[kProbChunk][4];
+#pragma unroll
+          for (int kk = 0; kk < kProbChunk; ++kk)
+#pragma unroll
+              for (int dir = 0; dir < 4; ++dir)
+                  pn[kk][dir] = (s0 + kk < S && inb[dir])
+                                    ? V.prob[(size_t)(s0 + kk) * HW + (y + DY[dir]) * W + (x + DX[dir])]
+                                    : 0.f;
+#pragma unroll
+          for (int kk = 0; kk < kProbChunk; ++kk) {
+            const int s = s0 + kk;
+            if (s >= S) break;
             const SrcLds &Ss = s_src[s];
             const float c0 = s_c0[s * kThreads + tid];
             float prev = 0.f;
             int np = 0;
 #pragma unroll
             for (int dir = 0; dir < 4; ++dir) {
-                const int qx = x + DX[dir], qy = y + DY[dir];
-                if (qx < 0 || qy < 0 || qx >= W || qy >= H) continue;
-                prev += V.prob[(size_t)s * HW + qy * W + qx];
+                if (!inb[dir]) continue;
+                prev += pn[kk][dir];
                 ++np;
             }
             prev = np ? prev / (float)np : 0.5f;
@@ -556,8 +600,12 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
             const float alpha = a1 / (a1 + a0v);
             V.prob[(size_t)s * HW + p] = alpha;
             const float q = view_weight(c, Ss, alpha, nh0, jtn);
+            // q waits in the sampled-list slot of source s: the draw below
+            // reads slot s before it writes list entry nlist <= s
+            s_list[s * kThreads + tid] = __float_as_uint(q);
             tot += q;
             if (q > 0.f) lastpos = s;
+          }
         }
         my_ncc += S;
 
@@ -575,7 +623,7 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
                 }
                 acc0 += 1.f * cc;
                 wsum += 1.f;
-                s_list[nlist * kThreads + tid] = (uint16_t)(s | (1 << 8));
+                s_list[nlist * kThreads + tid] = (uint32_t)(s | (1 << 8));
                 ++nlist;
             }
         } else {
@@ -590,8 +638,7 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
             float acc = 0.f;
             for (int s = 0; s < S; ++s) {
                 const SrcLds &Ss = s_src[s];
-                const float alpha = V.prob[(size_t)s * HW + p];
-                const float q = view_weight(c, Ss, alpha, nh0, jtn);
+                const float q = __uint_as_float(s_list[s * kThreads + tid]);
                 const float lo = acc;
                 acc = acc + q;
                 const float hi = s == lastpos ? INFINITY : acc;
@@ -608,7 +655,7 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
                     }
                     acc0 += (float)cnt * cc;
                     wsum += (float)cnt;
-                    s_list[nlist * kThreads + tid] = (uint16_t)(s | (cnt << 8));
+                    s_list[nlist * kThreads + tid] = (uint32_t)(s | (cnt << 8));
                     ++nlist;
                 }
             }
@@ -687,7 +734,7 @@ __global__ __launch_bounds__(kThreads) void k_halfstep(const ViewDesc *__restric
     __shared__ float s_ref[kLdsH * kLdsW];
     __shared__ SrcLds s_src[MAXS];
     __shared__ float s_c0[MAXS * kThreads];
-    __shared__ uint16_t s_list[MAXS * kThreads];
+    __shared__ uint32_t s_list[MAXS * kThreads];
     __shared__ unsigned int s_evals[2];
     __shared__ int s_tile;
     const int tid = threadIdx.x;

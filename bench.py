@@ -1,7 +1,7 @@
 """MVS throughput benchmark on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c3|c4]
-                    [--batched-step auto|1|2] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1|c2|c3|c4|c5]
+                    [--batched-step auto|1|2] [--tune-steps 1,2] [--no-cpu-baseline]
 
 Configurations (synthetic sites, seeded; images resident in HBM before
 timing; one JSON line on stdout):
@@ -20,11 +20,15 @@ c2 (default)  10 views of 1024x1024, every view a reference with 8 sources;
               ground truth are reported beside it.
 c3            32 reference views of ragged crop sizes (512..2048 per side),
               6..12 sources each, same passes; plain vs batched.
-c4            serving: Poisson arrivals (default 10 req/s) of per-view depth
+c4            serving: Poisson arrivals (default 20 req/s) of per-view depth
               requests from 4 sites (crops 1024..2048 per side, 8..11
               sources); continuous batching with HBM-footprint admission;
               p50/p99 latency and views/s, against a one-request-at-a-time
-              queue on the same arrival schedule.
+              queue on the same arrival schedule; `capacity` repeats both
+              forms with every request queued at t = 0.
+c5            40 views of 4096x4096, 20 sources, both passes, 0.3 m DSM.
+c1            2 views of 256x256, 1 source, photometric only (the case the
+              CPU oracle runs in full).
 """
 import argparse
 import json
@@ -272,7 +276,8 @@ def choose_step(ctx, views, a):
     if a.batched_step != "auto":
         return int(a.batched_step), {"window_step": int(a.batched_step), "source": "command line"}
     from vissatsatellitestereo_amd import autotune
-    st, rep = autotune.tune(ctx, views, iters=a.iters, radius=a.window_radius,
+    steps = tuple(int(x) for x in a.tune_steps.split(","))
+    st, rep = autotune.tune(ctx, views, iters=a.iters, radius=a.window_radius, steps=steps,
                             use_cache=not a.no_tune_cache)
     log("autotune", rep)
     return st, rep
@@ -428,6 +433,18 @@ def bench_serving(a):
         dsm[k] = ctx.fuse_dsm(grids[k], finished[k], depth_kind=0, min_measurements=3)
     queue_err = site_errors()
     log(f"queue: {len(reqs) / plain_wall:.2f} views/s", "dsm", queue_err)
+    # capacity: every request queued at t = 0, so neither form waits for
+    # arrivals (at the configured rate the served form keeps up with the
+    # arrivals and its views/s is the offered load)
+    ctx.set_params(window_step=step)
+    burst = ServeLoop(ctx).run_schedule([0.0] * len(reqs), reqs)
+    ctx.set_params(window_step=1)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for v in reqs:
+        ctx.solve([v], 0, want=("depth",))
+    plain_cap = len(reqs) / (time.perf_counter() - t0)
+    log(f"capacity: served {burst['views_per_s']:.2f} views/s, plain {plain_cap:.2f} views/s")
     out = {
         "metric": METRIC, "value": round(served["views_per_s"], 4), "unit": "views/s", "n_gpus": 1,
         "steps": len(reqs), "warmup": 1, "ms_per_step": round(1e3 * served["wall_s"] / len(reqs), 2),
@@ -449,6 +466,12 @@ def bench_serving(a):
                        "queue_p99": round(1e3 * float(np.percentile(lat, 99)), 1)},
         "plain_views_per_s": round(len(reqs) / plain_wall, 4),
         "batched_over_plain": round(served["views_per_s"] / (len(reqs) / plain_wall), 3),
+        "capacity": {"note": "all requests queued at t = 0",
+                     "served_views_per_s": round(burst["views_per_s"], 4),
+                     "plain_views_per_s": round(plain_cap, 4),
+                     "served_over_plain": round(burst["views_per_s"] / plain_cap, 3),
+                     "served_p50_ms": round(burst["p50_ms"], 1),
+                     "served_p99_ms": round(burst["p99_ms"], 1)},
         "serve_steps": served["steps"],
         "roofline": roofline(st, "c4"),
         "cpu_baseline": None,
@@ -483,7 +506,10 @@ def main():
     ap.add_argument("--config", default="c2", choices=("c1", "c2", "c3", "c4", "c5"))
     ap.add_argument("--iters", type=int, default=5)
     ap.add_argument("--window-radius", type=int, default=5)
-    ap.add_argument("--batched-step", default="auto", help="auto (autotuner), 1 or 2")
+    ap.add_argument("--batched-step", default="auto", help="auto (autotuner) or a fixed window step")
+    ap.add_argument("--tune-steps", default="1,2",
+                    help="window-step candidates of the autotuner (the batched configuration "
+                         "allows strides 1 and 2; wider ones only on request)")
     ap.add_argument("--no-tune-cache", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--forms", default="plain,batched", help="comma list of plain,batched")

@@ -104,14 +104,16 @@ def test_cost_fixed_hypotheses_c1(c1):
     assert err.max() <= COST_TOL, np.unravel_index(err.argmax(), err.shape)
 
 
-def test_cost_fixed_hypotheses_multisource_stride2(tiny):
-    p = _params(window_radius=4, window_step=2)
+@pytest.mark.parametrize("radius,step", [(4, 2), (5, 3)])
+def test_cost_fixed_hypotheses_multisource_strided(tiny, radius, step):
+    """Strided windows; r = 5 at step 3 visits offsets -5, -2, 1, 4 (no centre tap)."""
+    p = _params(window_radius=radius, window_step=step)
     ctx, ref, srcs = _setup(tiny, p, ref_i=1)
     d, n = _random_hyp(tiny, ref, seed=3)
     g = ctx.eval_cost(_id(tiny, ref), [_id(tiny, s) for s in srcs], tiny["dr"][ref], d, n, 0)
     o = _oracle(tiny, ref, srcs, p).cost_map(d.astype(np.float64), n.astype(np.float64), 0)
     err = np.abs(g - o)
-    print("cost stride2 max", err.max())
+    print(f"cost r{radius} step{step} max", err.max())
     assert err.max() <= COST_TOL
 
 

@@ -1,7 +1,9 @@
 """Autotuner for the batched form's window schedule.
 
-Knob: the tap stride inside the matching window (``window_step`` 1 or 2;
-stride 2 visits (floor(2r/2)+1)^2 taps instead of (2r+1)^2).
+Knob: the tap stride inside the matching window (``window_step``; the
+candidates are 1 and 2 by default, stride s visits (floor(2r/s)+1)^2 taps
+instead of (2r+1)^2).  Wider strides can be offered as candidates (``steps``);
+the accuracy proxy below decides whether they are accepted.
 
 Accuracy proxy (no ground truth needed): multi-view geometric consistency.
 Each candidate solves a calibration batch (up to ``calib_views`` reference
@@ -11,7 +13,7 @@ share of depths that at least two sources confirm photometrically and by
 forward-backward reprojection.  A candidate is accepted when its kept
 fraction is within ``max_drop`` of stride 1's; the fastest accepted one (wall
 time of both passes) wins.  Results are cached in a JSON file keyed by
-(height, width, sources, radius, iterations) size classes.
+(height, width, sources, radius, iterations) size classes and the candidate set.
 """
 import json
 import os
@@ -28,8 +30,9 @@ def size_class(n):
     return 1 << int(np.ceil(np.log2(max(1, n))))
 
 
-def key_of(shape, n_src, radius, iters):
-    return f"h{size_class(shape[0])}_w{size_class(shape[1])}_s{n_src}_r{radius}_i{iters}"
+def key_of(shape, n_src, radius, iters, steps=(1, 2)):
+    cand = "".join(str(int(s)) for s in steps)
+    return f"h{size_class(shape[0])}_w{size_class(shape[1])}_s{n_src}_r{radius}_i{iters}_c{cand}"
 
 
 def load_cache(path=CACHE):
@@ -56,7 +59,10 @@ def tune(ctx, views, iters, radius, steps=(1, 2), max_drop=0.02, calib_views=8,
     and geometric maps of the calibration views."""
     shape = ctx.sizes[int(views[0]["ref"])]
     n_src = max(len(v["srcs"]) for v in views)
-    key = key_of(shape, n_src, radius, iters)
+    steps = tuple(int(s) for s in steps if 1 <= int(s) <= radius)
+    if not steps or steps[0] != 1:
+        raise ValueError("window-step candidates must start with 1 (the accuracy reference)")
+    key = key_of(shape, n_src, radius, iters, steps)
     cache = load_cache(cache_path) if use_cache else {}
     if key in cache:
         return cache[key]["window_step"], dict(cache[key], cached=True)

@@ -23,7 +23,8 @@
 
 namespace vsm {
 
-constexpr int kGroupA = 4;      // candidate hypotheses per shared window pass (4 + 3)
+constexpr int kGroupH = 4;      // candidate hypotheses per shared window pass (4 + 3)
+constexpr int kGroupV = 4;      // sources per shared window pass in view selection
 constexpr int kFarChunk = 8;    // propagation candidates loaded together per direction
 constexpr int kProbChunk = 4;   // sources whose neighbour probabilities are loaded together
 
@@ -133,23 +134,6 @@ __device__ __forceinline__ SrcView view_of(const SrcLds &S)
     return v;
 }
 
-// Bilinear sample (DN) at anchor-relative (px, py), already inside the image.
-// The texel index iy*W + ix is formed with a signed 24-bit multiply-add on
-// anchor-relative integers plus the anchor's offset ay*W + ax (|relative
-// coordinates| < 2^23, image side <= 32768).
-__device__ __forceinline__ float sample_quad(const SrcView &S, float px, float py)
-{
-    const int ix = cvt_floor(px), iy = cvt_floor(py);
-    const float fx = __builtin_amdgcn_fractf(px), fy = __builtin_amdgcn_fractf(py);
-    const int idx = __mul24(iy, S.W) + (ix + S.base);
-    const uint32_t v = ((gquad_t)S.quad)[(unsigned)idx];
-    const float a = (float)(v & 0xFFu), b = (float)((v >> 8) & 0xFFu);
-    const float c = (float)((v >> 16) & 0xFFu), d = (float)(v >> 24);
-    const float top = fmaf(fx, b - a, a);
-    const float bot = fmaf(fx, d - c, c);
-    return fmaf(fy, bot - top, top);
-}
-
 // Image value of the texel containing anchor-relative (px, py) (inside the
 // image): the offset the source samples of a window are centred on.
 __device__ __forceinline__ float texel_at(const SrcView &S, float px, float py)
@@ -193,11 +177,58 @@ __device__ __forceinline__ RefStats ref_stats(const PMConst &c, const float *s_r
     return st;
 }
 
+// One bilinear tap split in two: tap_issue forms the anchor-relative source
+// position of column offset fi and starts the gather of its quad texel;
+// blend_quad interpolates the loaded texel (DN).  The texel index
+// iy*W + ix is formed with a signed 24-bit multiply-add on anchor-relative
+// integers plus the anchor's offset ay*W + ax (|relative coordinates| < 2^23,
+// image side <= 32768).
+template <bool EXACT, bool CLAMP>
+__device__ __forceinline__ void tap_issue(const SrcView &S, const Proj &P, float giu, float rx, float ry,
+                                          float rz, float fi, uint32_t &v, float &fx, float &fy)
+{
+    const float qx = fmaf(fi, P.hux, rx);
+    const float qy = fmaf(fi, P.huy, ry);
+    const float iz = EXACT ? __builtin_amdgcn_rcpf(fmaf(fi, P.huz, rz)) : fmaf(fi, giu, rz);
+    const float px = CLAMP ? __builtin_amdgcn_fmed3f(qx * iz, S.lox, S.hix) : qx * iz;
+    const float py = CLAMP ? __builtin_amdgcn_fmed3f(qy * iz, S.loy, S.hiy) : qy * iz;
+    const int ix = cvt_floor(px), iy = cvt_floor(py);
+    fx = __builtin_amdgcn_fractf(px);
+    fy = __builtin_amdgcn_fractf(py);
+    v = ((gquad_t)S.quad)[(unsigned)(__mul24(iy, S.W) + (ix + S.base))];
+}
+
+__device__ __forceinline__ float blend_quad(uint32_t v, float fx, float fy)
+{
+    const float a = (float)(v & 0xFFu), b = (float)((v >> 8) & 0xFFu);
+    const float c = (float)((v >> 16) & 0xFFu), d = (float)(v >> 24);
+    const float top = fmaf(fx, b - a, a);
+    const float bot = fmaf(fx, d - c, c);
+    return fmaf(fy, bot - top, top);
+}
+
 // Largest relative change of the homogeneous depth q.z across a window for
 // which the per-tap reciprocal is replaced by one Newton step from the exact
 // centre reciprocal: iz = iz0 (2 - qz iz0), relative error delta^2 <= 1e-6.
 // Satellite cameras (600 km range) give delta ~ 1e-5.
 constexpr float kDeltaMax = 1e-3f;
+
+// Row start of row offset dy for each hypothesis: q (or, without EXACT, the
+// linearised 1/q.z) at column offset -r.
+template <int G, bool EXACT>
+__device__ __forceinline__ void row_start(const Proj (&P)[G], const float (&iz0)[G],
+                                          const float (&giu)[G], const float (&giv)[G], float frad,
+                                          int dy, float (&rx)[G], float (&ry)[G], float (&rz)[G])
+{
+    const float fdy = (float)dy;
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        rx[k] = fmaf(fdy, P[k].hvx, fmaf(frad, P[k].hux, P[k].q0x));
+        ry[k] = fmaf(fdy, P[k].hvy, fmaf(frad, P[k].huy, P[k].q0y));
+        rz[k] = EXACT ? fmaf(fdy, P[k].hvz, fmaf(frad, P[k].huz, P[k].q0z))
+                      : fmaf(fdy, giv[k], fmaf(frad, giu[k], iz0[k]));
+    }
+}
 
 // Window pass of G plane hypotheses against the same source S: the bilateral
 // weight and the reference value of each tap are computed once and shared;
@@ -219,44 +250,61 @@ __device__ __forceinline__ void window_pass(const PMConst &c, const SrcView (&SV
         giv[k] = -iz0[k] * iz0[k] * P[k].hvz;
         ss[k] = 0.f; sss[k] = 0.f; srs[k] = 0.f;
     }
-    for (int j = 0; j < nt; ++j) {
-        const int dy = -rad + j * st;
-        const float fdy = (float)dy;
-        float rx[G], ry[G], rz[G];
+    // Taps in row-major order as one sequence; the gathers of tap t+1
+    // are issued before tap t is accumulated, with two register sets
+    // used in turn.  The issue past the last tap repeats the last tap.
+    const int T = nt * nt;
+    int ii = 0, jj = 0;                       // next tap to issue
+    float rx[G], ry[G], rz[G];
+    row_start<G, EXACT>(P, iz0, giu, giv, frad, -rad, rx, ry, rz);
+    uint32_t va[G], vb[G];
+    float xa[G], ya[G], xb[G], yb[G];
+    // tok: values the gathers must wait for (empty asm statements tie
+    // them to the tap position, so the compiler keeps the issue after
+    // the accumulation of the set being reloaded)
+    auto issue = [&](uint32_t (&v)[G], float (&fx)[G], float (&fy)[G], const float (&tok)[G]) {
+        float fi = (float)(ii * st);
+#pragma unroll
+        for (int k = 0; k < G; ++k) asm volatile("" : "+v"(fi) : "v"(tok[k]));
+#pragma unroll
+        for (int k = 0; k < G; ++k)
+            tap_issue<EXACT, CLAMP>(SV[k], P[k], giu[k], rx[k], ry[k], rz[k], fi, v[k], fx[k], fy[k]);
+        if (ii + 1 < nt) {
+            ++ii;
+        } else if (jj + 1 < nt) {
+            ii = 0;
+            ++jj;
+            row_start<G, EXACT>(P, iz0, giu, giv, frad, -rad + jj * st, rx, ry, rz);
+        }
+    };
+    int ic = 0, jc = 0;                       // next tap to accumulate
+    auto consume = [&](const uint32_t (&v)[G], const float (&fx)[G], const float (&fy)[G]) {
+        const int dx = -rad + ic * st, dy = -rad + jc * st;
+        const float r = s_ref[lbase + dy * kLdsW + dx] - R.rc;
+        const float w = tap_weight(c, r, dx, dy);
+        const float wr = w * r;
 #pragma unroll
         for (int k = 0; k < G; ++k) {
-            rx[k] = fmaf(fdy, P[k].hvx, fmaf(frad, P[k].hux, P[k].q0x));
-            ry[k] = fmaf(fdy, P[k].hvy, fmaf(frad, P[k].huy, P[k].q0y));
-            rz[k] = EXACT ? fmaf(fdy, P[k].hvz, fmaf(frad, P[k].huz, P[k].q0z))
-                          : fmaf(fdy, giv[k], fmaf(frad, giu[k], iz0[k]));
+            const float sv = blend_quad(v[k], fx[k], fy[k]) - sc[k];
+            const float ws = w * sv;
+            ss[k] += ws;
+            sss[k] = fmaf(ws, sv, sss[k]);
+            srs[k] = fmaf(wr, sv, srs[k]);
         }
-        const float *row = s_ref + lbase + dy * kLdsW;
+        if (++ic == nt) { ic = 0; ++jc; }
+    };
+    issue(va, xa, ya, ss);
+    int t = 0;
 #pragma unroll 1
-        for (int i = 0; i < nt; ++i) {
-            const int dx = -rad + i * st;
-            const float fi = (float)(i * st);
-            const float r = row[dx] - R.rc;
-            const float w = tap_weight(c, r, dx, dy);
-            const float wr = w * r;
-#pragma unroll
-            for (int k = 0; k < G; ++k) {
-                const SrcView &S = SV[k];
-                const float qx = fmaf(fi, P[k].hux, rx[k]);
-                const float qy = fmaf(fi, P[k].huy, ry[k]);
-                const float iz = EXACT ? __builtin_amdgcn_rcpf(fmaf(fi, P[k].huz, rz[k]))
-                                       : fmaf(fi, giu[k], rz[k]);
-                // hypotheses are finite (NaN-free), so med3 is a plain clamp;
-                // windows proven inside the image skip it
-                const float px = CLAMP ? __builtin_amdgcn_fmed3f(qx * iz, S.lox, S.hix) : qx * iz;
-                const float py = CLAMP ? __builtin_amdgcn_fmed3f(qy * iz, S.loy, S.hiy) : qy * iz;
-                const float sv = sample_quad(S, px, py) - sc[k];
-                const float ws = w * sv;
-                ss[k] += ws;
-                sss[k] = fmaf(ws, sv, sss[k]);
-                srs[k] = fmaf(wr, sv, srs[k]);
-            }
-        }
+    for (; t + 1 < T; t += 2) {
+        issue(vb, xb, yb, ss);
+        consume(va, xa, ya);
+        // set a's next gathers come after its last use (no register copy
+        // at the loop edge, which would wait for them)
+        issue(va, xa, ya, ss);
+        consume(vb, xb, yb);
     }
+    if (t < T) consume(va, xa, ya);
 }
 
 // NCC cost of G plane hypotheses.  Each hypothesis' source samples are
@@ -421,6 +469,63 @@ __device__ __forceinline__ T pick8(int k, T v0, T v1, T v2, T v3, T v4, T v5, T 
     return r;
 }
 
+// Costs of the plane (d, a, b) against the G sources s0 .. s0+G-1 in one
+// shared window pass, stored in s_c0[s][tid].
+template <int G>
+__device__ __forceinline__ void select_group(const PMConst &c, const SrcLds *s_src, int s0, float du,
+                                             float dv, float d, float a, float b, const float *s_ref,
+                                             int lbase, const RefStats &R, float *s_c0, int tid)
+{
+    Proj P[G];
+    float cg[G];
+    const SrcLds *SS[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        P[k] = make_proj(s_src[s0 + k], du, dv, d, a, b);
+        SS[k] = &s_src[s0 + k];
+    }
+    ncc_cost_group<G>(c, SS, P, s_ref, lbase, R, cg);
+#pragma unroll
+    for (int k = 0; k < G; ++k) s_c0[(s0 + k) * kThreads + tid] = cg[k];
+}
+
+// Candidate hypotheses K0+1 .. K0+G against source Ss in one shared window
+// pass; acc[k] += cnt * cost (photometric, plus the geometric term in pass 1).
+template <int G, int K0>
+__device__ __forceinline__ void eval_hyp_group(const PMConst &c, const SrcLds &Ss, float cnt, int pass,
+                                               float du, float dv, const float (&hd)[kNHyp],
+                                               const float (&ak)[kNHyp - 1],
+                                               const float (&bk)[kNHyp - 1], const float *s_ref,
+                                               int lbase, const RefStats &R, float (&acc)[kNHyp - 1])
+{
+    Proj P[G];
+    float ph[G];
+    const SrcLds *SS[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        P[k] = make_proj(Ss, du, dv, hd[K0 + k + 1], ak[K0 + k], bk[K0 + k]);
+        SS[k] = &Ss;
+    }
+    ncc_cost_group<G>(c, SS, P, s_ref, lbase, R, ph);
+#pragma unroll
+    for (int k = 0; k < G; ++k) acc[K0 + k] += cnt * total_cost(c, Ss, P[k], ph[k], pass, du, dv);
+}
+
+// All candidate hypotheses 1 .. kNHyp-1 in groups of (at most) G.
+template <int G, int K0>
+__device__ __forceinline__ void eval_hyps(const PMConst &c, const SrcLds &Ss, float cnt, int pass,
+                                          float du, float dv, const float (&hd)[kNHyp],
+                                          const float (&ak)[kNHyp - 1], const float (&bk)[kNHyp - 1],
+                                          const float *s_ref, int lbase, const RefStats &R,
+                                          float (&acc)[kNHyp - 1])
+{
+    if constexpr (K0 < kNHyp - 1) {
+        constexpr int g = (kNHyp - 1 - K0) < G ? (kNHyp - 1 - K0) : G;
+        eval_hyp_group<g, K0>(c, Ss, cnt, pass, du, dv, hd, ak, bk, s_ref, lbase, R, acc);
+        eval_hyps<G, K0 + g>(c, Ss, cnt, pass, du, dv, hd, ak, bk, s_ref, lbase, R, acc);
+    }
+}
+
 // Common tile prologue: reference tile (replicated border) into LDS and the
 // tile-centred source matrices.
 template <int MAXS>
@@ -530,33 +635,21 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
             const float j2 = J[2] * nh0[0] + J[5] * nh0[1] + J[8] * nh0[2];
             jtn = sqrtf(j0 * j0 + j1 * j1 + j2 * j2);
         }
-        // costs of the current plane against every source, up to four
+        // costs of the current plane against every source, up to kGroupV
         // sources sharing one window pass
         for (int s0 = 0; s0 < S;) {
             const int left = S - s0;
-            if (left >= 7 || left == 4 || left == 5) {
-                Proj P[4];
-                float c4[4];
-#pragma unroll
-                for (int k = 0; k < 4; ++k) P[k] = make_proj(s_src[s0 + k], du, dv, hd[0], a0, b0);
-                const SrcLds *const SS[4] = {&s_src[s0], &s_src[s0 + 1], &s_src[s0 + 2], &s_src[s0 + 3]};
-                ncc_cost_group<4>(c, SS, P, s_ref, lbase, R, c4);
-#pragma unroll
-                for (int k = 0; k < 4; ++k) s_c0[(s0 + k) * kThreads + tid] = c4[k];
+            if (kGroupV >= 4 && (left >= 7 || left == 4 || left == 5)) {
+                select_group<4>(c, s_src, s0, du, dv, hd[0], a0, b0, s_ref, lbase, R, s_c0, tid);
                 s0 += 4;
-            } else if (left == 3 || left == 6) {
-                Proj P[3];
-                float c3[3];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) P[k] = make_proj(s_src[s0 + k], du, dv, hd[0], a0, b0);
-                const SrcLds *const SS[3] = {&s_src[s0], &s_src[s0 + 1], &s_src[s0 + 2]};
-                ncc_cost_group<3>(c, SS, P, s_ref, lbase, R, c3);
-#pragma unroll
-                for (int k = 0; k < 3; ++k) s_c0[(s0 + k) * kThreads + tid] = c3[k];
+            } else if (kGroupV >= 3 && (left == 3 || left == 6)) {
+                select_group<3>(c, s_src, s0, du, dv, hd[0], a0, b0, s_ref, lbase, R, s_c0, tid);
                 s0 += 3;
+            } else if (kGroupV >= 2 && left >= 2) {
+                select_group<2>(c, s_src, s0, du, dv, hd[0], a0, b0, s_ref, lbase, R, s_c0, tid);
+                s0 += 2;
             } else {
-                const Proj P = make_proj(s_src[s0], du, dv, hd[0], a0, b0);
-                s_c0[s0 * kThreads + tid] = ncc_cost(c, s_src[s0], P, s_ref, lbase, R);
+                select_group<1>(c, s_src, s0, du, dv, hd[0], a0, b0, s_ref, lbase, R, s_c0, tid);
                 s0 += 1;
             }
         }
@@ -623,7 +716,7 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
                 }
                 acc0 += 1.f * cc;
                 wsum += 1.f;
-                s_list[nlist * kThreads + tid] = (uint32_t)(s | (1 << 8));
+                s_list[s * kThreads + tid] = 1u;
                 ++nlist;
             }
         } else {
@@ -655,14 +748,19 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
                     }
                     acc0 += (float)cnt * cc;
                     wsum += (float)cnt;
-                    s_list[nlist * kThreads + tid] = (uint32_t)(s | (cnt << 8));
                     ++nlist;
                 }
+                // slot s now holds the draw count of source s (0: not drawn)
+                s_list[s * kThreads + tid] = (uint32_t)cnt;
             }
         }
 
         // ---- evaluate the other seven hypotheses -------------------------
         // One pass over the window per sampled source evaluates all seven.
+        // Sources are visited in ascending order by the whole wave (a lane
+        // that did not draw source s is masked off), so every gather of a
+        // pass reads one source image; each lane still accumulates its drawn
+        // sources in ascending order.
         float ak[kNHyp - 1], bk[kNHyp - 1], acc[kNHyp - 1];
 #pragma unroll
         for (int k = 0; k < kNHyp - 1; ++k) {
@@ -670,34 +768,12 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
             bk[k] = -hn1[k + 1] / hn2[k + 1];
             acc[k] = 0.f;
         }
-        for (int j = 0; j < nlist; ++j) {
-            const uint32_t e = s_list[j * kThreads + tid];
-            const SrcLds &Ss = s_src[e & 0xFFu];
-            const float cnt = (float)(e >> 8);
-            {   // hypotheses 1..kGroupA
-                Proj P[kGroupA];
-                float ph[kGroupA];
-#pragma unroll
-                for (int k = 0; k < kGroupA; ++k) P[k] = make_proj(Ss, du, dv, hd[k + 1], ak[k], bk[k]);
-                const SrcLds *const SS[kGroupA] = {&Ss, &Ss, &Ss, &Ss};
-                ncc_cost_group<kGroupA>(c, SS, P, s_ref, lbase, R, ph);
-#pragma unroll
-                for (int k = 0; k < kGroupA; ++k)
-                    acc[k] += cnt * total_cost(c, Ss, P[k], ph[k], pass, du, dv);
-            }
-            {   // hypotheses kGroupA+1..7
-                constexpr int GB = kNHyp - 1 - kGroupA;
-                Proj P[GB];
-                float ph[GB];
-#pragma unroll
-                for (int k = 0; k < GB; ++k)
-                    P[k] = make_proj(Ss, du, dv, hd[kGroupA + 1 + k], ak[kGroupA + k], bk[kGroupA + k]);
-                const SrcLds *const SS[GB] = {&Ss, &Ss, &Ss};
-                ncc_cost_group<GB>(c, SS, P, s_ref, lbase, R, ph);
-#pragma unroll
-                for (int k = 0; k < GB; ++k)
-                    acc[kGroupA + k] += cnt * total_cost(c, Ss, P[k], ph[k], pass, du, dv);
-            }
+        for (int s = 0; s < S; ++s) {
+            const uint32_t e = s_list[s * kThreads + tid];
+            if (e == 0u) continue;
+            const SrcLds &Ss = s_src[s];
+            const float cnt = (float)e;
+            eval_hyps<kGroupH, 0>(c, Ss, cnt, pass, du, dv, hd, ak, bk, s_ref, lbase, R, acc);
         }
         my_ncc += (kNHyp - 1) * nlist;
         if (pass == 1) my_geo += (kNHyp - 1) * nlist;
@@ -726,7 +802,7 @@ constexpr int kXcd = 8;   // MI355X: 8 XCDs, workgroups dispatched round-robin a
 // order tiles are taken in never changes a result: tiles are independent
 // within a half-step.
 template <int MAXS>
-__global__ __launch_bounds__(kThreads) void k_halfstep(const ViewDesc *__restrict__ views,
+__global__ __launch_bounds__(kThreads, 2) void k_halfstep(const ViewDesc *__restrict__ views,
                                                        const Tile *__restrict__ tiles, int ntiles,
                                                        PMConst c, Counters *counters,
                                                        unsigned int *work)

@@ -123,14 +123,25 @@ __device__ __forceinline__ int cvt_floor(float x)
 struct SrcView {
     float lox, hix, loy, hiy;
     int W, base;
+    int W4, base4;               // row pitch and anchor offset in bytes
     const uint32_t *quad;
 };
 
+// Every lane of a wave reads the same source in a window pass, so the texture
+// pointer, row pitch and anchor offset go to scalar registers and each gather
+// is a scalar base plus a 32-bit byte offset (4 * texel index < 2^32 for
+// images of at most 2^30 texels).
 __device__ __forceinline__ SrcView view_of(const SrcLds &S)
 {
     SrcView v;
     v.lox = S.lox; v.hix = S.hix; v.loy = S.loy; v.hiy = S.hiy;
-    v.W = S.W; v.base = S.base; v.quad = S.quad;
+    v.W = S.W; v.base = S.base;
+    v.W4 = __builtin_amdgcn_readfirstlane(S.W * 4);
+    v.base4 = __builtin_amdgcn_readfirstlane(S.base * 4);
+    const uint64_t a = (uint64_t)S.quad;
+    const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a);
+    const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a >> 32));
+    v.quad = (const uint32_t *)(lo | (hi << 32));
     return v;
 }
 
@@ -151,20 +162,30 @@ __device__ __forceinline__ float tap_weight(const PMConst &c, float r, int dx, i
 
 struct RefStats {
     float rc, sw, inv_sw, mr, var_r;
+    const float *wt;             // tap weights in LDS (stride kThreads), or null
 };
 
-__device__ __forceinline__ RefStats ref_stats(const PMConst &c, const float *s_ref, int lbase)
+// Windows of at most kWTab taps keep each pixel's tap weights in LDS (written
+// once per half-step by ref_stats, read by every window pass of the pixel;
+// 36 KB per workgroup for 6 x 6 taps).
+constexpr int kWTab = 36;
+
+__device__ __forceinline__ RefStats ref_stats(const PMConst &c, const float *s_ref, int lbase,
+                                              float *wt = nullptr)
 {
     RefStats st;
     st.rc = s_ref[lbase];
+    st.wt = wt;
     float sw = 0.f, sr = 0.f, srr = 0.f;
+    int t = 0;
     for (int j = 0; j < c.ntaps; ++j) {
         const int dy = -c.radius + j * c.step;
         const float *row = s_ref + lbase + dy * kLdsW;
-        for (int i = 0; i < c.ntaps; ++i) {
+        for (int i = 0; i < c.ntaps; ++i, ++t) {
             const int dx = -c.radius + i * c.step;
             const float r = row[dx] - st.rc;
             const float w = tap_weight(c, r, dx, dy);
+            if (wt) wt[t * kThreads] = w;
             sw += w;
             sr = fmaf(w, r, sr);
             srr = fmaf(w * r, r, srr);
@@ -195,7 +216,8 @@ __device__ __forceinline__ void tap_issue(const SrcView &S, const Proj &P, float
     const int ix = cvt_floor(px), iy = cvt_floor(py);
     fx = __builtin_amdgcn_fractf(px);
     fy = __builtin_amdgcn_fractf(py);
-    v = ((gquad_t)S.quad)[(unsigned)(__mul24(iy, S.W) + (ix + S.base))];
+    const unsigned off = (unsigned)(__mul24(iy, S.W4) + ((ix << 2) + S.base4));
+    v = *(gquad_t)((const __attribute__((address_space(1))) char *)S.quad + off);
 }
 
 __device__ __forceinline__ float blend_quad(uint32_t v, float fx, float fy)
@@ -234,7 +256,7 @@ __device__ __forceinline__ void row_start(const Proj (&P)[G], const float (&iz0)
 // weight and the reference value of each tap are computed once and shared;
 // the G gathers per tap are independent.  EXACT selects the per-tap
 // reciprocal (fallback when some lane's window exceeds kDeltaMax).
-template <int G, bool EXACT, bool CLAMP>
+template <int G, bool EXACT, bool CLAMP, bool WT>
 __device__ __forceinline__ void window_pass(const PMConst &c, const SrcView (&SV)[G],
                                             const Proj (&P)[G], const float (&iz0)[G],
                                             const float (&sc)[G], const float *s_ref, int lbase,
@@ -277,11 +299,12 @@ __device__ __forceinline__ void window_pass(const PMConst &c, const SrcView (&SV
             row_start<G, EXACT>(P, iz0, giu, giv, frad, -rad + jj * st, rx, ry, rz);
         }
     };
-    int ic = 0, jc = 0;                       // next tap to accumulate
+    int ic = 0, jc = 0, tc = 0;               // next tap to accumulate
     auto consume = [&](const uint32_t (&v)[G], const float (&fx)[G], const float (&fy)[G]) {
         const int dx = -rad + ic * st, dy = -rad + jc * st;
         const float r = s_ref[lbase + dy * kLdsW + dx] - R.rc;
-        const float w = tap_weight(c, r, dx, dy);
+        const float w = WT ? R.wt[tc * kThreads] : tap_weight(c, r, dx, dy);
+        ++tc;
         const float wr = w * r;
 #pragma unroll
         for (int k = 0; k < G; ++k) {
@@ -341,12 +364,19 @@ __device__ __forceinline__ void ncc_cost_group(const PMConst &c, const SrcLds *c
         const float my = fr * (fabsf(jyx) + fabsf(jyy)) * (1.f + 2.f * delta) + 1.f;
         edge = edge || !(cx - mx >= S.lox && cx + mx <= S.hix && cy - my >= S.loy && cy + my <= S.hiy);
     }
-    if (__ballot(far) != 0ull)
-        window_pass<G, true, true>(c, SV, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
-    else if (__ballot(edge) != 0ull)
-        window_pass<G, false, true>(c, SV, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
-    else
-        window_pass<G, false, false>(c, SV, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
+    if (__ballot(far) != 0ull) {
+        window_pass<G, true, true, false>(c, SV, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
+    } else if (R.wt != nullptr) {
+        if (__ballot(edge) != 0ull)
+            window_pass<G, false, true, true>(c, SV, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
+        else
+            window_pass<G, false, false, true>(c, SV, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
+    } else {
+        if (__ballot(edge) != 0ull)
+            window_pass<G, false, true, false>(c, SV, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
+        else
+            window_pass<G, false, false, false>(c, SV, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
+    }
 #pragma unroll
     for (int k = 0; k < G; ++k) {
         const float ms = ss[k] * R.inv_sw;
@@ -548,7 +578,7 @@ __device__ __forceinline__ void tile_prologue(const ViewDesc &V, const Tile &t, 
 template <int MAXS>
 __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, const PMConst &c,
                                               float *s_ref, SrcLds *s_src, float *s_c0,
-                                              uint32_t *s_list, unsigned int &my_ncc,
+                                              uint32_t *s_list, float *s_w, unsigned int &my_ncc,
                                               unsigned int &my_geo)
 {
     const int iter = V.step >> 1, color = V.step & 1, pass = V.pass;
@@ -624,7 +654,8 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
         }
 
         // ---- view selection on the current hypothesis --------------------
-        const RefStats R = ref_stats(c, s_ref, lbase);
+        const bool wtab = MAXS <= 16 && c.ntaps * c.ntaps <= kWTab;
+        const RefStats R = ref_stats(c, s_ref, lbase, wtab ? s_w + tid : nullptr);
         const float a0 = -hn0[0] / hn2[0], b0 = -hn1[0] / hn2[0];
         const float nh0[3] = {hn0[0], hn1[0], hn2[0]};
         float jtn;
@@ -811,6 +842,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_halfstep(const ViewDesc *__rest
     __shared__ SrcLds s_src[MAXS];
     __shared__ float s_c0[MAXS * kThreads];
     __shared__ uint32_t s_list[MAXS * kThreads];
+    __shared__ float s_w[(MAXS <= 16 ? kWTab : 1) * kThreads];
     __shared__ unsigned int s_evals[2];
     __shared__ int s_tile;
     const int tid = threadIdx.x;
@@ -833,7 +865,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_halfstep(const ViewDesc *__rest
             const Tile t = tiles[ti];
             const ViewDesc &V = views[t.view];
             if (!V.active || V.step >= V.num_steps) continue;     // uniform per workgroup
-            halfstep_tile<MAXS>(V, t, c, s_ref, s_src, s_c0, s_list, my_ncc, my_geo);
+            halfstep_tile<MAXS>(V, t, c, s_ref, s_src, s_c0, s_list, s_w, my_ncc, my_geo);
         }
     }
     if (counters) {

@@ -186,6 +186,15 @@ def test_lockstep_photometric_multisource(tiny):
         assert r["frac_cost_err"] <= 1e-3, r
 
 
+def test_lockstep_photometric_strided(tiny):
+    """r = 5 at window step 2: the 36-tap window of the batched form (tap
+    weights kept in LDS) against the oracle, half-step by half-step."""
+    rep, _ = _lockstep(tiny, _params(window_radius=5, window_step=2), 0, 3, 0, 6)
+    for r in rep:
+        assert r["decided_mismatch"] <= 1e-3, r
+        assert r["frac_cost_err"] <= 1e-3, r
+
+
 def test_full_solve_accuracy_and_oracle_agreement(c1):
     p = _params()
     ctx, ref, srcs = _setup(c1, p)
@@ -264,6 +273,16 @@ def test_lockstep_geometric_multisource(tiny):
     names = tiny["names"]
     srcs = [n for n in names if n != names[0]]
     p = _params(window_radius=4)
+    rep, st = _lockstep(tiny, p, 0, None, 1, 4, src_depths_gpu=_src_depths(tiny, srcs))
+    for r in rep:
+        assert r["decided_mismatch"] <= 1e-3, r
+        assert r["frac_cost_err"] <= 1e-3, r
+
+
+def test_lockstep_geometric_strided(tiny):
+    names = tiny["names"]
+    srcs = [n for n in names if n != names[0]]
+    p = _params(window_radius=5, window_step=2)
     rep, st = _lockstep(tiny, p, 0, None, 1, 4, src_depths_gpu=_src_depths(tiny, srcs))
     for r in rep:
         assert r["decided_mismatch"] <= 1e-3, r

@@ -76,6 +76,16 @@ def tiny(site_dir):
     return sites.build("tiny", sites.tiny_spec(), site_dir)
 
 
+@pytest.fixture(scope="module")
+def many(site_dir):
+    """19 views of 96x80: 18 sources per reference, the kernel instantiation
+    for more than 16 sources."""
+    from vissatsatellitestereo_amd import synth
+    spec = synth.SceneSpec(width=96, height=80, n_views=19, off_nadir=(5.0, 30.0), n_boxes=4,
+                           n_points=800)
+    return sites.build("many", spec, site_dir)
+
+
 def _random_hyp(site, ref, seed=7):
     h, w = site["imgs"][ref].shape
     rng = np.random.default_rng(seed)
@@ -115,6 +125,27 @@ def test_cost_fixed_hypotheses_multisource_strided(tiny, radius, step):
     err = np.abs(g - o)
     print(f"cost r{radius} step{step} max", err.max())
     assert err.max() <= COST_TOL
+
+
+def test_cost_fixed_hypotheses_18_sources(many):
+    p = _params(window_radius=5, window_step=2)
+    ctx, ref, srcs = _setup(many, p)
+    assert len(srcs) == 18
+    d, n = _random_hyp(many, ref, seed=5)
+    g = ctx.eval_cost(_id(many, ref), [_id(many, s) for s in srcs], many["dr"][ref], d, n, 0)
+    o = _oracle(many, ref, srcs, p).cost_map(d.astype(np.float64), n.astype(np.float64), 0)
+    err = np.abs(g - o)
+    print("cost 18 sources max", err.max())
+    assert err.max() <= COST_TOL
+
+
+def test_lockstep_photometric_18_sources(many):
+    """Half-steps of the kernel for up to 32 sources (no LDS tap-weight table)
+    against the oracle."""
+    rep, _ = _lockstep(many, _params(window_radius=5, window_step=2), 0, None, 0, 4)
+    for r in rep:
+        assert r["decided_mismatch"] <= 1e-3, r
+        assert r["frac_cost_err"] <= 1e-3, r
 
 
 def test_init_matches_oracle(c1):

@@ -759,17 +759,28 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
                 tk[4 * b + 0] = u[0] * tot; tk[4 * b + 1] = u[1] * tot;
                 tk[4 * b + 2] = u[2] * tot; tk[4 * b + 3] = u[3] * tot;
             }
+            // samples beyond num_samples are never counted
+#pragma unroll
+            for (int k = 0; k < kMaxSamples; ++k)
+                if (k >= c.num_samples) tk[k] = INFINITY;
+            // Source s takes the samples in [lo_s, hi_s): lo_s is the sum of
+            // the weights before s, hi_s = lo_s + q_s, and the last source of
+            // positive weight takes everything from its lo up (hi = inf for
+            // it and for every later source, which then counts 0).  The count
+            // is #(t < hi_s) - #(t < lo_s); #(t < lo_s) is the previous
+            // source's #(t < hi).
             float acc = 0.f;
+            int below = 0;
             for (int s = 0; s < S; ++s) {
                 const SrcLds &Ss = s_src[s];
                 const float q = __uint_as_float(s_list[s * kThreads + tid]);
-                const float lo = acc;
                 acc = acc + q;
-                const float hi = s == lastpos ? INFINITY : acc;
-                int cnt = 0;
+                const float hi = s >= lastpos ? INFINITY : acc;
+                int nb = 0;
 #pragma unroll
-                for (int k = 0; k < kMaxSamples; ++k)
-                    cnt += (k < c.num_samples && tk[k] >= lo && tk[k] < hi) ? 1 : 0;
+                for (int k = 0; k < kMaxSamples; ++k) nb += tk[k] < hi ? 1 : 0;
+                const int cnt = nb - below;
+                below = nb;
                 if (cnt > 0) {
                     float cc = s_c0[s * kThreads + tid];
                     if (pass == 1) {

@@ -86,6 +86,17 @@ def many(site_dir):
     return sites.build("many", spec, site_dir)
 
 
+@pytest.fixture(scope="module")
+def ragged(site_dir):
+    """4 views whose sizes are not multiples of the 32 x 16 tile: partial
+    tiles at the right and bottom edges."""
+    from vissatsatellitestereo_amd import synth
+    spec = synth.SceneSpec(n_views=4, off_nadir=(10.0, 25.0), azimuths=(0.0, 90.0, 180.0, 270.0),
+                           sizes=[(150, 100), (136, 94), (121, 110), (144, 87)], n_boxes=5,
+                           n_points=800)
+    return sites.build("ragged", spec, site_dir)
+
+
 def _random_hyp(site, ref, seed=7):
     h, w = site["imgs"][ref].shape
     rng = np.random.default_rng(seed)
@@ -266,6 +277,30 @@ def test_batched_equals_single(tiny):
         assert np.array_equal(one["depth"], b["depth"])
         assert np.array_equal(one["normal"], b["normal"])
         assert np.array_equal(g["depth"], b["depth"])
+
+
+def test_lockstep_partial_tiles(ragged):
+    """Reference 150 x 100 (partial tiles on two edges), window step 2."""
+    assert ragged["imgs"][ragged["names"][0]].shape == (100, 150)
+    rep, _ = _lockstep(ragged, _params(window_radius=5, window_step=2), 0, None, 0, 4)
+    for r in rep:
+        assert r["decided_mismatch"] <= 1e-3, r
+        assert r["frac_cost_err"] <= 1e-3, r
+
+
+def test_batched_equals_single_partial_tiles(ragged):
+    p = _params(window_radius=5, window_step=2, num_iterations=2)
+    ctx, _, _ = _setup(ragged, p)
+    views = []
+    for r in ragged["names"]:
+        srcs = [n for n in ragged["names"] if n != r]
+        views.append(dict(ref=_id(ragged, r), srcs=[_id(ragged, s) for s in srcs],
+                          depth_range=ragged["dr"][r]))
+    batch = ctx.solve(views, 0, use_graph=True)
+    for v, b in zip(views, batch):
+        one = ctx.solve([v], 0)[0]
+        assert np.array_equal(one["depth"], b["depth"])
+        assert np.array_equal(one["normal"], b["normal"])
 
 
 def test_geometric_pass_and_filter(tiny):

@@ -167,19 +167,21 @@ def roofline(st, cfg="c2"):
     avg_ms = st["halfstep_ms"] / st["halfstep_launches"]
     taps = st["taps"] / st["halfstep_launches"]
     ach = FLOPS_PER_TAP * taps / (avg_ms * 1e-3) / 1e12
-    traffic = None
+    traffic, source = None, None
     # HBM traffic per launch comes from a committed counter profile of the
     # same configuration (none for the others)
     tfile = os.path.join(ROOT, "profiles", f"k_halfstep_{cfg}_counters.json")
     if os.path.exists(tfile):
         t = json.load(open(tfile))
-        traffic = {"bytes_per_launch": round(t["fetch_bytes"] + t["write_bytes"]),
-                   "source": "profiles/k_halfstep_c2_counters.md (FETCH_SIZE + WRITE_SIZE per "
-                             "dispatch, calibrated on an elementwise kernel of known bytes; "
-                             "c2 batched, window step 2)"}
+        traffic = round(t["fetch_bytes"] + t["write_bytes"])
+        source = (f"profiles/k_halfstep_{cfg}_counters.md: FETCH_SIZE + WRITE_SIZE per dispatch, "
+                  "calibrated on an elementwise kernel of known bytes (batched, window step 2)")
     return {"bound": "valu", "kernel": "k_halfstep", "achieved": round(ach, 3),
             "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / FP32_PEAK_TFLOPS, 4),
-            "traffic": traffic, "avg_launch_ms": round(avg_ms, 4), "taps_per_launch": int(taps),
+            "traffic": traffic, "traffic_unit": "bytes per launch", "traffic_source": source,
+            "bound_note": "vector FP32 (VALU): a gather/stencil kernel, neither MFMA nor HBM "
+                          "work; peak = MI355X FP32 157.3 TFLOP/s",
+            "avg_launch_ms": round(avg_ms, 4), "taps_per_launch": int(taps),
             "flops_per_tap": FLOPS_PER_TAP}
 
 

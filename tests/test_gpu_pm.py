@@ -412,3 +412,50 @@ def test_get_depth_map_round_trip(tiny):
         assert np.array_equal(ctx.get_depth_map(701, kind=0), out[0]["depth"])
     finally:
         ctx.close()
+
+
+def test_api_rejects_bad_input_and_stays_usable(tiny):
+    """Invalid parameters and requests raise (nothing reaches a kernel) and
+    leave the context as it was: a valid solve afterwards matches a fresh
+    context's."""
+    from vissatsatellitestereo_amd._lib import VsmError
+    from vissatsatellitestereo_amd.mvs import MvsContext
+    names = tiny["names"]
+    ids = [800 + i for i in range(len(names))]
+    v = dict(ref=ids[0], srcs=ids[1:], depth_range=tiny["dr"][names[0]])
+    ctx = MvsContext(device=0, window_radius=4, num_iterations=2)
+    try:
+        for i, n in enumerate(names):
+            ctx.add_image(ids[i], tiny["imgs"][n], tiny["P4"][n])
+        for bad in (dict(window_step=5), dict(window_radius=0), dict(window_radius=99),
+                    dict(num_samples=17), dict(num_samples=0), dict(n_far=0)):
+            with pytest.raises(VsmError):
+                ctx.set_params(**bad)
+        assert ctx.params.window_radius == 4 and ctx.params.window_step == 1
+        bad_views = [dict(v, ref=999), dict(v, srcs=[999]), dict(v, srcs=[ids[0]]),
+                     dict(v, srcs=list(range(900, 933))), dict(v, depth_range=(5.0, 5.0)),
+                     dict(v, depth_range=(float("nan"), 10.0))]
+        for bv in bad_views:
+            with pytest.raises(VsmError):
+                ctx.solve([bv], 0, want=("depth",))
+        with pytest.raises(VsmError):
+            ctx.add_image(900, tiny["imgs"][names[0]], np.full((4, 4), np.nan))
+        with pytest.raises(VsmError):
+            ctx.set_depth_map(ids[0], np.ones((3, 3), np.float32), kind=0)
+        got = ctx.solve([v], 0, want=("depth",))[0]["depth"]
+    finally:
+        ctx.close()
+    fresh = MvsContext(device=0, window_radius=4, num_iterations=2)
+    try:
+        for i, n in enumerate(names):
+            fresh.add_image(ids[i], tiny["imgs"][n], tiny["P4"][n])
+        ref = fresh.solve([v], 0, want=("depth",))[0]["depth"]
+    finally:
+        fresh.close()
+    assert np.array_equal(got, ref)
+    small = MvsContext(device=0, hbm_budget_bytes=1 << 16)
+    try:
+        with pytest.raises(VsmError):
+            small.add_image(1, tiny["imgs"][names[0]], tiny["P4"][names[0]])
+    finally:
+        small.close()

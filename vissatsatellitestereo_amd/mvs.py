@@ -151,6 +151,8 @@ class MvsContext:
                             v.get("init_normal"))
             reqs[i] = r
             keep.append(k)
+            if int(v["ref"]) not in self.sizes:
+                raise VsmError(-2, f"unknown reference image {int(v['ref'])}")
             h, w = self.sizes[int(v["ref"])]
             res = {}
             if "depth" in want:

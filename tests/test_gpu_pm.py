@@ -159,6 +159,24 @@ def test_lockstep_photometric_18_sources(many):
         assert r["frac_cost_err"] <= 1e-3, r
 
 
+def test_lockstep_photometric_12_sources(many):
+    """Half-steps of the kernel for 9..16 sources against the oracle."""
+    rep, _ = _lockstep(many, _params(window_radius=5, window_step=2), 0, 12, 0, 4)
+    for r in rep:
+        assert r["decided_mismatch"] <= 1e-3, r
+        assert r["frac_cost_err"] <= 1e-3, r
+
+
+def test_lockstep_geometric_18_sources(many):
+    names = many["names"]
+    srcs = [n for n in names if n != names[0]]
+    rep, _ = _lockstep(many, _params(window_radius=5, window_step=2), 0, None, 1, 2,
+                       src_depths_gpu=_src_depths(many, srcs))
+    for r in rep:
+        assert r["decided_mismatch"] <= 1e-3, r
+        assert r["frac_cost_err"] <= 1e-3, r
+
+
 def test_init_matches_oracle(c1):
     p = _params()
     ctx, ref, srcs = _setup(c1, p)

@@ -81,6 +81,8 @@ struct vsm_ctx {
     std::string err;
     Counters *d_counters = nullptr;
     unsigned int *d_work = nullptr;        // per-XCD tile cursors of the persistent kernel
+    float *d_c0 = nullptr;                 // per-workgroup view-selection costs (> 16 sources)
+    size_t c0_floats = 0;
     int grid[3] = {0, 0, 0};               // resident workgroups per MAXS variant
     bool timing = false;
     double halfstep_ms = 0.0;
@@ -513,12 +515,34 @@ int grid_for(vsm_ctx *c, int maxs)
     return c->grid[k];
 }
 
+// The half-step kernel for more than 16 sources keeps each workgroup's
+// view-selection costs (32 sources x 256 threads) in global memory; sized for
+// the largest grid it is launched with.  Called before any launch (and before
+// graph capture).
+int ensure_c0(vsm_ctx *c, int maxs)
+{
+    if (maxs <= 16) return VSM_OK;
+    const size_t need = (size_t)grid_for(c, maxs) * 32 * 256;
+    if (c->c0_floats >= need) return VSM_OK;
+    if (c->d_c0) dfree(c, c->d_c0);
+    c->d_c0 = nullptr;
+    c->c0_floats = 0;
+    const int rc = dalloc(c, &c->d_c0, need);
+    if (rc) return rc;
+    c->c0_floats = need;
+    return VSM_OK;
+}
+
 int record_halfstep(vsm_ctx *c, int maxs, const ViewDesc *dviews, const Tile *dtiles, int ntiles,
                     const PMConst &pc, bool timed)
 {
+    {
+        const int rc = ensure_c0(c, maxs);
+        if (rc) return rc;
+    }
     if (timed) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     HIPCHK(c, launch_halfstep(maxs, dviews, dtiles, ntiles, pc, c->d_counters, c->d_work,
-                              grid_for(c, maxs), c->stream));
+                              grid_for(c, maxs), c->d_c0, c->stream));
     if (timed) {
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
         HIPCHK(c, hipEventSynchronize(c->ev1));
@@ -559,7 +583,8 @@ int run_views(vsm_ctx *c, std::vector<ViewState> &views, const std::vector<const
     Tile *dtiles = nullptr;
     int rc = dalloc(c, &dviews, n);
     if (!rc) rc = dalloc(c, &dtiles, tiles.size());
-    if (rc) { dfree(c, dviews); return rc; }
+    if (!rc) rc = ensure_c0(c, maxs);     // before any capture
+    if (rc) { dfree(c, dviews); dfree(c, dtiles); return rc; }
     std::vector<ViewDesc> hv(n);
     for (int v = 0; v < n; ++v) hv[v] = views[v].desc;
     auto cleanup = [&]() { dfree(c, dviews); dfree(c, dtiles); };
@@ -611,7 +636,7 @@ int run_views(vsm_ctx *c, std::vector<ViewState> &views, const std::vector<const
         hipError_t ce = hipSuccess;
         for (int s = 0; s < nsteps && ce == hipSuccess; ++s) {
             ce = launch_halfstep(maxs, dviews, dtiles, ntiles, pc, c->d_counters, c->d_work,
-                                 grid_for(c, maxs), c->stream);
+                                 grid_for(c, maxs), c->d_c0, c->stream);
             if (ce == hipSuccess) ce = launch_advance(dviews, n, c->stream);
         }
         hipError_t ee = hipStreamEndCapture(c->stream, &graph);
@@ -1000,6 +1025,7 @@ static int debug_state(vsm_ctx *c, const vsm_view_req *req, int pass, int mode, 
     rc = dalloc(c, &dv, 1);
     if (!rc) rc = dalloc(c, &dt, tiles.size());
     if (!rc && mode == 2) rc = dalloc(c, &dm, hw);
+    if (!rc && mode == 1) rc = ensure_c0(c, V.n_src);
     const PMConst pc = make_const(c->params);
     std::vector<float4> h(hw);
     if (!rc) {
@@ -1016,7 +1042,7 @@ static int debug_state(vsm_ctx *c, const vsm_view_req *req, int pass, int mode, 
         if (mode == 0) HIPCHK(c, launch_init(dv, dt, nt, pc, 1, c->stream));
         else if (mode == 1)
             HIPCHK(c, launch_halfstep(V.n_src, dv, dt, nt, pc, c->d_counters, c->d_work,
-                                      grid_for(c, V.n_src), c->stream));
+                                      grid_for(c, V.n_src), c->d_c0, c->stream));
         else HIPCHK(c, launch_filter(V.n_src, dv, dt, nt, pc, dm, c->stream));
         HIPCHK(c, hipMemcpyAsync(h.data(), vs.hyp, hw * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
         if (cost && mode != 2) HIPCHK(c, hipMemcpyAsync(cost, vs.cost, hw * 4, hipMemcpyDeviceToHost, c->stream));

@@ -7,7 +7,7 @@ namespace vsm {
 
 hipError_t launch_halfstep(int maxs, const ViewDesc *views, const Tile *tiles, int ntiles,
                            const PMConst &c, Counters *counters, unsigned int *work, int grid,
-                           hipStream_t st);
+                           float *c0_scratch, hipStream_t st);
 // Resident workgroups of the persistent half-step kernel on this device.
 int halfstep_grid(int maxs);
 hipError_t launch_init(const ViewDesc *views, const Tile *tiles, int ntiles, const PMConst &c,

@@ -654,7 +654,7 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
         }
 
         // ---- view selection on the current hypothesis --------------------
-        const bool wtab = MAXS <= 16 && c.ntaps * c.ntaps <= kWTab;
+        const bool wtab = c.ntaps * c.ntaps <= kWTab;
         const RefStats R = ref_stats(c, s_ref, lbase, wtab ? s_w + tid : nullptr);
         const float a0 = -hn0[0] / hn2[0], b0 = -hn1[0] / hn2[0];
         const float nh0[3] = {hn0[0], hn1[0], hn2[0]};
@@ -703,12 +703,16 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
                   pn[kk][dir] = (s0 + kk < S && inb[dir])
                                     ? V.prob[(size_t)(s0 + kk) * HW + (y + DY[dir]) * W + (x + DX[dir])]
                                     : 0.f;
+          float c0v[kProbChunk];
+#pragma unroll
+          for (int kk = 0; kk < kProbChunk; ++kk)
+              c0v[kk] = s0 + kk < S ? s_c0[(s0 + kk) * kThreads + tid] : 0.f;
 #pragma unroll
           for (int kk = 0; kk < kProbChunk; ++kk) {
             const int s = s0 + kk;
             if (s >= S) break;
             const SrcLds &Ss = s_src[s];
-            const float c0 = s_c0[s * kThreads + tid];
+            const float c0 = c0v[kk];
             float prev = 0.f;
             int np = 0;
 #pragma unroll
@@ -774,6 +778,7 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
             for (int s = 0; s < S; ++s) {
                 const SrcLds &Ss = s_src[s];
                 const float q = __uint_as_float(s_list[s * kThreads + tid]);
+                const float c0s = s_c0[s * kThreads + tid];
                 acc = acc + q;
                 const float hi = s >= lastpos ? INFINITY : acc;
                 int nb = 0;
@@ -782,7 +787,7 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
                 const int cnt = nb - below;
                 below = nb;
                 if (cnt > 0) {
-                    float cc = s_c0[s * kThreads + tid];
+                    float cc = c0s;
                     if (pass == 1) {
                         const Proj P = make_proj(Ss, du, dv, hd[0], a0, b0);
                         cc = total_cost(c, Ss, P, cc, pass, du, dv);
@@ -847,13 +852,15 @@ template <int MAXS>
 __global__ __launch_bounds__(kThreads, 2) void k_halfstep(const ViewDesc *__restrict__ views,
                                                        const Tile *__restrict__ tiles, int ntiles,
                                                        PMConst c, Counters *counters,
-                                                       unsigned int *work)
+                                                       unsigned int *work, float *c0g)
 {
     __shared__ float s_ref[kLdsH * kLdsW];
     __shared__ SrcLds s_src[MAXS];
-    __shared__ float s_c0[MAXS * kThreads];
+    // kernels for more than 16 sources keep the view-selection costs in a
+    // per-workgroup global scratch, which leaves LDS room for the tap weights
+    __shared__ float s_c0[MAXS > 16 ? 1 : MAXS * kThreads];
     __shared__ uint32_t s_list[MAXS * kThreads];
-    __shared__ float s_w[(MAXS <= 16 ? kWTab : 1) * kThreads];
+    __shared__ float s_w[kWTab * kThreads];
     __shared__ unsigned int s_evals[2];
     __shared__ int s_tile;
     const int tid = threadIdx.x;
@@ -876,7 +883,9 @@ __global__ __launch_bounds__(kThreads, 2) void k_halfstep(const ViewDesc *__rest
             const Tile t = tiles[ti];
             const ViewDesc &V = views[t.view];
             if (!V.active || V.step >= V.num_steps) continue;     // uniform per workgroup
-            halfstep_tile<MAXS>(V, t, c, s_ref, s_src, s_c0, s_list, s_w, my_ncc, my_geo);
+            halfstep_tile<MAXS>(V, t, c, s_ref, s_src,
+                                MAXS > 16 ? c0g + (size_t)blockIdx.x * (MAXS * kThreads) : s_c0,
+                                s_list, s_w, my_ncc, my_geo);
         }
     }
     if (counters) {
@@ -1087,18 +1096,20 @@ __global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)
 // ---------------------------------------------------------------------------
 hipError_t launch_halfstep(int maxs, const ViewDesc *views, const Tile *tiles, int ntiles,
                            const PMConst &c, Counters *counters, unsigned int *work, int grid,
-                           hipStream_t st)
+                           float *c0_scratch, hipStream_t st)
 {
     if (ntiles <= 0) return hipSuccess;
+    // more than 16 sources: c0_scratch holds grid x 32 x kThreads floats
+    if (maxs > 16 && !c0_scratch) return hipErrorInvalidValue;
     hipError_t e = launch_fill_u32(work, kXcd, 0u, st);
     if (e != hipSuccess) return e;
     grid = max(kXcd, min(grid, ((ntiles + kXcd - 1) / kXcd) * kXcd));
     if (maxs <= 8)
-        k_halfstep<8><<<grid, kThreads, 0, st>>>(views, tiles, ntiles, c, counters, work);
+        k_halfstep<8><<<grid, kThreads, 0, st>>>(views, tiles, ntiles, c, counters, work, c0_scratch);
     else if (maxs <= 16)
-        k_halfstep<16><<<grid, kThreads, 0, st>>>(views, tiles, ntiles, c, counters, work);
+        k_halfstep<16><<<grid, kThreads, 0, st>>>(views, tiles, ntiles, c, counters, work, c0_scratch);
     else
-        k_halfstep<32><<<grid, kThreads, 0, st>>>(views, tiles, ntiles, c, counters, work);
+        k_halfstep<32><<<grid, kThreads, 0, st>>>(views, tiles, ntiles, c, counters, work, c0_scratch);
     return hipGetLastError();
 }
 

@@ -536,9 +536,14 @@ __device__ __forceinline__ void eval_hyp_group(const PMConst &c, const SrcLds &S
         P[k] = make_proj(Ss, du, dv, hd[K0 + k + 1], ak[K0 + k], bk[K0 + k]);
         SS[k] = &Ss;
     }
+    // geometric term first: its depth gathers overlap the window pass
+    float gt[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k)
+        gt[k] = pass == 1 ? c.geom_reg * fminf(geo_error(c, Ss, P[k], du, dv), c.geom_max) : 0.f;
     ncc_cost_group<G>(c, SS, P, s_ref, lbase, R, ph);
 #pragma unroll
-    for (int k = 0; k < G; ++k) acc[K0 + k] += cnt * total_cost(c, Ss, P[k], ph[k], pass, du, dv);
+    for (int k = 0; k < G; ++k) acc[K0 + k] += cnt * (pass == 1 ? ph[k] + gt[k] : ph[k]);
 }
 
 // All candidate hypotheses 1 .. kNHyp-1 in groups of (at most) G.

@@ -13,6 +13,12 @@
 // at 600 km camera range.  Compiled with -ffp-contract=off: every fused
 // multiply-add is an explicit fmaf, so recomputed quantities are bit-identical.
 //
+// Throughput structure of a window pass (the ~80 % of the time spent in
+// evaluating plane hypotheses): the whole wave reads one source image (the
+// texture base is a scalar), the gathers of tap t+1 are issued before tap t
+// is accumulated, and for windows of at most 36 taps the bilateral weights
+// come from a per-pixel LDS table.  The kernel is pinned to 2 waves per SIMD.
+//
 // Semantics follow the oracle (oracle/pm_oracle.c): bilaterally weighted NCC,
 // adaptive red-black propagation (best stored cost among 2k+1 steps in each of
 // four directions), random and perturbed hypotheses, Monte-Carlo view

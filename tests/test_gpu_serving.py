@@ -125,3 +125,36 @@ def test_served_result_becomes_device_depth_and_fuses(tiny):
                for t, n in zip(tickets, tiny["names"])]
     assert np.array_equal(dsm, fo.fuse(np.stack(views_o), 3), equal_nan=True)
     ctx.close()
+
+
+@pytest.fixture(scope="module")
+def many(site_dir):
+    from vissatsatellitestereo_amd import synth
+    spec = synth.SceneSpec(width=96, height=80, n_views=19, off_nadir=(5.0, 30.0), n_boxes=4,
+                           n_points=800)
+    return sites.build("many", spec, site_dir)
+
+
+def test_served_with_18_sources_equals_batched(many):
+    """The kernel for more than 16 sources (view-selection costs in the
+    context's global scratch, tap weights in LDS at window step 2) gives the
+    same results served with staggered arrivals as in one batched solve."""
+    from vissatsatellitestereo_amd.serving import ServeLoop
+    views = _views(many)[:3]
+    assert all(len(v["srcs"]) == 18 for v in views)
+    ctx = _ctx(many, window_radius=5, window_step=2, num_iterations=2)
+    try:
+        loop = ServeLoop(ctx, want=("depth", "normal", "cost"))
+        t0 = loop.submit(views[1])
+        loop.step()
+        t1 = loop.submit(views[0])
+        loop.step()
+        t2 = loop.submit(views[2])
+        loop.drain()
+        served = [loop.result(t1), loop.result(t0), loop.result(t2)]
+        batched = ctx.solve(views, 0, want=("depth", "normal", "cost"))
+        for i in range(3):
+            for k in ("depth", "normal", "cost"):
+                assert np.array_equal(served[i][k], batched[i][k]), (i, k)
+    finally:
+        ctx.close()

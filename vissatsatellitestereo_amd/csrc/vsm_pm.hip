@@ -965,6 +965,36 @@ __global__ __launch_bounds__(kThreads) void k_eval_cost(const ViewDesc *__restri
     }
 }
 
+// Filter rule of the plane (d, a, b) against the G sources s0 .. s0+G-1,
+// their costs from one shared window pass: a source is consistent when
+// 1 - cost >= min NCC, its triangulation angle passes and, in the geometric
+// pass, the forward-backward error passes; bit s of m is set for it.
+template <int G>
+__device__ __forceinline__ void filter_group(const PMConst &c, int pass, const SrcLds *s_src, int s0,
+                                             float du, float dv, float d, float a, float b,
+                                             const float *s_ref, int lbase, const RefStats &R,
+                                             uint32_t &m, int &count)
+{
+    Proj P[G];
+    float ph[G];
+    const SrcLds *SS[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        P[k] = make_proj(s_src[s0 + k], du, dv, d, a, b);
+        SS[k] = &s_src[s0 + k];
+    }
+    ncc_cost_group<G>(c, SS, P, s_ref, lbase, R, ph);
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        const SrcLds &Ss = s_src[s0 + k];
+        if (!(1.f - ph[k] >= c.f_min_ncc)) continue;
+        if (!(Ss.tri >= c.f_min_tri)) continue;
+        if (pass == 1 && !(geo_error(c, Ss, P[k], du, dv) <= c.f_geom_max)) continue;
+        m |= 1u << (s0 + k);
+        ++count;
+    }
+}
+
 // Filter: count consistent sources; zero the hypothesis when too few.
 template <int MAXS>
 __global__ __launch_bounds__(kThreads) void k_filter(const ViewDesc *__restrict__ views,
@@ -973,8 +1003,10 @@ __global__ __launch_bounds__(kThreads) void k_filter(const ViewDesc *__restrict_
 {
     __shared__ float s_ref[kLdsH * kLdsW];
     __shared__ SrcLds s_src[MAXS];
+    __shared__ float s_w[kWTab * kThreads];
     const Tile t = tiles[blockIdx.x];
     const ViewDesc &V = views[t.view];
+    const bool wtab = c.ntaps * c.ntaps <= kWTab;
     tile_prologue<MAXS>(V, t, s_ref, s_src);
     __syncthreads();
     for (int k = threadIdx.x; k < kTileW * kTileH; k += kThreads) {
@@ -992,17 +1024,25 @@ __global__ __launch_bounds__(kThreads) void k_filter(const ViewDesc *__restrict_
         {
             const int lbase = (ty + kMaxRadius) * kLdsW + tx + kMaxRadius;
             const float du = (float)(tx - kTileW / 2), dv = (float)(ty - kTileH / 2);
-            const RefStats R = ref_stats(c, s_ref, lbase);
+            const RefStats R = ref_stats(c, s_ref, lbase, wtab ? s_w + threadIdx.x : nullptr);
             const float a = -h.y / h.w, b = -h.z / h.w;
-            for (int s = 0; s < V.n_src; ++s) {
-                const SrcLds &Ss = s_src[s];
-                const Proj P = make_proj(Ss, du, dv, h.x, a, b);
-                const float ph = ncc_cost(c, Ss, P, s_ref, lbase, R);
-                if (!(1.f - ph >= c.f_min_ncc)) continue;
-                if (!(Ss.tri >= c.f_min_tri)) continue;
-                if (V.pass == 1 && !(geo_error(c, Ss, P, du, dv) <= c.f_geom_max)) continue;
-                m |= 1u << s;
-                ++count;
+            // up to four sources share a window pass (as in view selection)
+            const int S = V.n_src;
+            for (int s0 = 0; s0 < S;) {
+                const int left = S - s0;
+                if (left >= 7 || left == 4 || left == 5) {
+                    filter_group<4>(c, V.pass, s_src, s0, du, dv, h.x, a, b, s_ref, lbase, R, m, count);
+                    s0 += 4;
+                } else if (left == 3 || left == 6) {
+                    filter_group<3>(c, V.pass, s_src, s0, du, dv, h.x, a, b, s_ref, lbase, R, m, count);
+                    s0 += 3;
+                } else if (left == 2) {
+                    filter_group<2>(c, V.pass, s_src, s0, du, dv, h.x, a, b, s_ref, lbase, R, m, count);
+                    s0 += 2;
+                } else {
+                    filter_group<1>(c, V.pass, s_src, s0, du, dv, h.x, a, b, s_ref, lbase, R, m, count);
+                    s0 += 1;
+                }
             }
         }
         if (mask_out) mask_out[p] = m;

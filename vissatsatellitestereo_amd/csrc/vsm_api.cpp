@@ -422,10 +422,24 @@ int check_params(vsm_ctx *c, const vsm_params &p)
     return VSM_OK;
 }
 
+// Tiles are listed in blocks of 8 x 8 tiles (256 x 128 pixels), blocks row-major.
+// The persistent half-step kernel gives each XCD a contiguous share of the list and
+// its 64 resident workgroups take consecutive tiles, so at any moment they cover one
+// square block, whose footprint in each source image is smaller than that of a
+// 64-tile strip: +5.6 % on 20 sources of 4096^2, +1 % on 8 of 1024^2 (8 x 16,
+// 16 x 8 and 16 x 16 blocks measured the same).  Tiles are independent within a
+// half-step and the random numbers are keyed by view and pixel, so the order never
+// changes a result.
+constexpr int kBlockTiles = 8;
+
 void tiles_for(const ViewDesc &V, int view_index, std::vector<Tile> &out)
 {
-    for (int y0 = 0; y0 < V.H; y0 += kTileH)
-        for (int x0 = 0; x0 < V.W; x0 += kTileW) out.push_back(Tile{view_index, x0, y0, 0});
+    const int tw = (V.W + kTileW - 1) / kTileW, th = (V.H + kTileH - 1) / kTileH;
+    for (int by = 0; by < th; by += kBlockTiles)
+        for (int bx = 0; bx < tw; bx += kBlockTiles)
+            for (int ty = by; ty < std::min(th, by + kBlockTiles); ++ty)
+                for (int tx = bx; tx < std::min(tw, bx + kBlockTiles); ++tx)
+                    out.push_back(Tile{view_index, tx * kTileW, ty * kTileH, 0});
 }
 
 size_t state_bytes(int W, int H, int S)

@@ -175,7 +175,8 @@ def roofline(st, cfg="c2"):
         t = json.load(open(tfile))
         traffic = round(t["fetch_bytes"] + t["write_bytes"])
         source = (f"profiles/k_halfstep_{cfg}_counters.md: FETCH_SIZE + WRITE_SIZE per dispatch, "
-                  "calibrated on an elementwise kernel of known bytes (batched, window step 2)")
+                  "calibrated on an elementwise kernel of known bytes (batched, window step 2)"
+                  + (f"; {t['calibration']}" if "calibration" in t else ""))
     return {"bound": "valu", "kernel": "k_halfstep", "achieved": round(ach, 3),
             "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / FP32_PEAK_TFLOPS, 4),
             "traffic": traffic, "traffic_unit": "bytes per launch", "traffic_source": source,

@@ -44,6 +44,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 METRIC = "MVS views/s + p50/p99 per-view latency, batched vs plain, 1 MI355X"
 FP32_PEAK_TFLOPS = 157.3      # MI355X vector FP32 (published)
+HBM_PEAK_GBPS = 8000.0        # MI355X HBM3E (published)
 FLOPS_PER_TAP = 30            # homography 12, divide 3, bilinear 8, accumulate 7
 
 
@@ -167,7 +168,7 @@ def roofline(st, cfg="c2"):
     avg_ms = st["halfstep_ms"] / st["halfstep_launches"]
     taps = st["taps"] / st["halfstep_launches"]
     ach = FLOPS_PER_TAP * taps / (avg_ms * 1e-3) / 1e12
-    traffic, source = None, None
+    traffic, source, profiled = None, None, None
     # HBM traffic per launch comes from a committed counter profile of the
     # same configuration (none for the others)
     tfile = os.path.join(ROOT, "profiles", f"k_halfstep_{cfg}_counters.json")
@@ -177,9 +178,16 @@ def roofline(st, cfg="c2"):
         source = (f"profiles/k_halfstep_{cfg}_counters.md: FETCH_SIZE + WRITE_SIZE per dispatch, "
                   "calibrated on an elementwise kernel of known bytes (batched, window step 2)"
                   + (f"; {t['calibration']}" if "calibration" in t else ""))
+        # the profile's traffic over this run's live launch time, against the
+        # 8 TB/s HBM peak; VALU issue and lane figures as the profile read them
+        gbps = traffic / (avg_ms * 1e-3) / 1e9
+        profiled = {"hbm_gbps": round(gbps, 1), "hbm_frac": round(gbps / HBM_PEAK_GBPS, 4),
+                    "valu_busy_pct": round(t["valubusy"], 1) if "valubusy" in t else None,
+                    "valu_lanes_active_pct": round(t["valuutil"], 1) if "valuutil" in t else None}
     return {"bound": "valu", "kernel": "k_halfstep", "achieved": round(ach, 3),
             "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / FP32_PEAK_TFLOPS, 4),
             "traffic": traffic, "traffic_unit": "bytes per launch", "traffic_source": source,
+            "profiled": profiled,
             "bound_note": "vector FP32 (VALU): a gather/stencil kernel, neither MFMA nor HBM "
                           "work; peak = MI355X FP32 157.3 TFLOP/s",
             "avg_launch_ms": round(avg_ms, 4), "taps_per_launch": int(taps),

@@ -456,6 +456,10 @@ def test_api_rejects_bad_input_and_stays_usable(tiny):
         for bv in bad_views:
             with pytest.raises(VsmError):
                 ctx.solve([bv], 0, want=("depth",))
+        with pytest.raises(VsmError):          # empty batch
+            ctx.solve([], 0, want=("depth",))
+        with pytest.raises(VsmError):          # fusion of no views
+            ctx.fuse_dsm((0.0, 0.0, 0.5, 0.5, 8, 8), [])
         with pytest.raises(VsmError):
             ctx.add_image(900, tiny["imgs"][names[0]], np.full((4, 4), np.nan))
         with pytest.raises(VsmError):

@@ -462,6 +462,9 @@ def test_api_rejects_bad_input_and_stays_usable(tiny):
             ctx.fuse_dsm((0.0, 0.0, 0.5, 0.5, 8, 8), [])
         with pytest.raises(VsmError):
             ctx.add_image(900, tiny["imgs"][names[0]], np.full((4, 4), np.nan))
+        for shape in ((2, 32769), (1, 64)):    # wider than 32768 px; a single row
+            with pytest.raises(VsmError):
+                ctx.add_image(901, np.zeros(shape, np.uint8), tiny["P4"][names[0]])
         with pytest.raises(VsmError):
             ctx.set_depth_map(ids[0], np.ones((3, 3), np.float32), kind=0)
         got = ctx.solve([v], 0, want=("depth",))[0]["depth"]

@@ -147,6 +147,31 @@ static inline void proj(const double *M, double x, double y, double d, double q[
     q[2] = M[8] * x + M[9] * y + M[10] + M[11] * d;
 }
 
+/* Homogeneous source position of the window tap (dx, dy) of pixel (x, y)
+ * under the plane of depth d and slopes (a, b): the plane depth at the tap is
+ * d + a dx + b dy, projected through M. */
+static inline void tap_proj(const double *M, int x, int y, double d, double a, double b, int dx,
+                            int dy, double t[3])
+{
+    const double dd = d + a * dx + b * dy;
+    proj(M, x + dx, y + dy, dd, t);
+}
+
+/* Source pixel (not clamped) of the window tap (dx, dy) of pixel (x, y) under
+ * the hypothesis (d, n) against source s; returns 0 when the tap's homogeneous
+ * depth is not positive.  Exported so the tap geometry can be checked against
+ * the plane-induced homography of debuggers/sfm_debug_warping_subarea.py:45-57. */
+int orc_tap_coord(const orc_view *v, int s, int x, int y, double d, const double n[3], int dx,
+                  int dy, double out[2])
+{
+    const double a = -n[0] / n[2], b = -n[1] / n[2];
+    double t[3];
+    tap_proj(v->M + 12 * s, x, y, d, a, b, dx, dy, t);
+    out[0] = t[0] / t[2];
+    out[1] = t[1] / t[2];
+    return t[2] > 0.0;
+}
+
 /* 1 - bilaterally weighted NCC between the reference window at (x,y) and its
  * image in source s under the plane (d, n). */
 double orc_photo_cost(const orc_view *v, const orc_params *p, int x, int y, double d,
@@ -176,9 +201,8 @@ double orc_photo_cost(const orc_view *v, const orc_params *p, int x, int y, doub
             const int dx = -r + i * st;
             const int rx = x + dx < 0 ? 0 : (x + dx >= W ? W - 1 : x + dx);
             const double rv = v->ref[(size_t)ry * W + rx] / 255.0 - rc;
-            const double dd = d + a * dx + b * dy;
             double t[3];
-            proj(M, x + dx, y + dy, dd, t);
+            tap_proj(M, x, y, d, a, b, dx, dy, t);
             const double px = clampd(t[0] / t[2], 0.0, Ws - 1);
             const double py = clampd(t[1] / t[2], 0.0, Hs - 1);
             const double sv = bilinear_u8(src, Ws, Hs, px, py) / 255.0 - sc;
@@ -315,6 +339,60 @@ void orc_reset_state(const orc_view *v, double *cost, double *prob)
     for (size_t i = 0; i < HW * v->n_src; ++i) prob[i] = 0.5;
 }
 
+/* Monte-Carlo view selection: num_samples draws of a source with probability
+ * q[s] / tot (inverse CDF over the sources in index order; a draw that
+ * rounding leaves past the last boundary goes to the last source of positive
+ * weight); cnt[s] = number of draws of s.  tot must be the sum of q in index
+ * order.  With no positive weight every source counts once. */
+static void draw_sources(const orc_view *v, const orc_params *p, uint32_t pixel, uint32_t code,
+                         const double *q, double tot, int *cnt)
+{
+    const int S = v->n_src;
+    for (int s = 0; s < S; ++s) cnt[s] = 0;
+    if (!(tot > 0.0)) {
+        for (int s = 0; s < S; ++s) cnt[s] = 1;
+        return;
+    }
+    int last_pos = 0;
+    for (int s = 0; s < S; ++s) if (q[s] > 0.0) last_pos = s;
+    for (int k = 0; k < p->num_samples; ++k) {
+        double ub[4];
+        rng_block(v, p, pixel, code, 2u + (uint32_t)(k >> 2), ub);
+        const double target = ub[k & 3] * tot;
+        double acc = 0.0;
+        int pick = -1;
+        for (int s = 0; s < S; ++s) {
+            acc += q[s];
+            if (target < acc) { pick = s; break; }
+        }
+        if (pick < 0) pick = last_pos;
+        cnt[pick] += 1;
+    }
+}
+
+/* The draw of one pixel for given weights (exported for the statistical
+ * checks of the sampler). */
+void orc_draw(const orc_view *v, const orc_params *p, uint32_t pixel, uint32_t code,
+              const double *q, int32_t *cnt)
+{
+    double tot = 0.0;
+    int c[ORC_MAX_SRC];
+    for (int s = 0; s < v->n_src; ++s) tot += q[s];
+    draw_sources(v, p, pixel, code, q, tot, c);
+    for (int s = 0; s < v->n_src; ++s) cnt[s] = c[s];
+}
+
+void orc_random_normal(const orc_params *p, double u1, double u2, double n[3])
+{
+    random_normal(p, u1, u2, n);
+}
+
+void orc_perturb_normal(const orc_params *p, const double n[3], double amax, double u1, double u2,
+                        double out[3])
+{
+    perturb_normal(p, n, amax, u1, u2, out);
+}
+
 static void propagate(const double *depth, const double *normal, int W, int xq, int yq, int x,
                       int y, double *d, double n[3])
 {
@@ -408,26 +486,7 @@ void orc_halfstep(const orc_view *v, const orc_params *p, int pass, int iter, in
                 tot += q[s];
             }
             int cnt[ORC_MAX_SRC];
-            for (int s = 0; s < S; ++s) cnt[s] = 0;
-            if (!(tot > 0.0)) {
-                for (int s = 0; s < S; ++s) cnt[s] = 1;
-            } else {
-                int last_pos = 0;
-                for (int s = 0; s < S; ++s) if (q[s] > 0.0) last_pos = s;
-                for (int k = 0; k < p->num_samples; ++k) {
-                    double ub[4];
-                    rng_block(v, p, (uint32_t)i, code, 2u + (uint32_t)(k >> 2), ub);
-                    const double target = ub[k & 3] * tot;
-                    double acc = 0.0;
-                    int pick = -1;
-                    for (int s = 0; s < S; ++s) {
-                        acc += q[s];
-                        if (target < acc) { pick = s; break; }
-                    }
-                    if (pick < 0) pick = last_pos;
-                    cnt[pick] += 1;
-                }
-            }
+            draw_sources(v, p, (uint32_t)i, code, q, tot, cnt);
             double wsum = 0.0;
             for (int s = 0; s < S; ++s) wsum += cnt[s];
             double best_c = 0.0;

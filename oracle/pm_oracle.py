@@ -38,6 +38,19 @@ def lib():
         L.orc_photo_cost.restype = ctypes.c_double
         L.orc_photo_cost.argtypes = [P(OrcView), P(OrcParams), ctypes.c_int, ctypes.c_int,
                                      ctypes.c_double, P(ctypes.c_double), ctypes.c_int]
+        L.orc_tap_coord.restype = ctypes.c_int
+        L.orc_tap_coord.argtypes = [P(OrcView), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_double, P(ctypes.c_double), ctypes.c_int,
+                                    ctypes.c_int, P(ctypes.c_double)]
+        L.orc_draw.restype = None
+        L.orc_draw.argtypes = [P(OrcView), P(OrcParams), ctypes.c_uint32, ctypes.c_uint32,
+                               P(ctypes.c_double), P(ctypes.c_int32)]
+        L.orc_random_normal.restype = None
+        L.orc_random_normal.argtypes = [P(OrcParams), ctypes.c_double, ctypes.c_double,
+                                        P(ctypes.c_double)]
+        L.orc_perturb_normal.restype = None
+        L.orc_perturb_normal.argtypes = [P(OrcParams), P(ctypes.c_double), ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, P(ctypes.c_double)]
         L.orc_geo_error.restype = ctypes.c_double
         L.orc_geo_error.argtypes = [P(OrcView), P(OrcParams), ctypes.c_int, ctypes.c_int,
                                     ctypes.c_double, ctypes.c_int]
@@ -251,6 +264,27 @@ class OracleView:
         lib().orc_cost_map(ctypes.byref(v), ctypes.byref(p), pass_, _dptr(d), _dptr(n), _dptr(out))
         return out.reshape(self.S, self.H, self.W)
 
+    def tap_coord(self, s, x, y, d, n, dx, dy):
+        """Source pixel (px, py) of window tap (dx, dy) of pixel (x, y) under the
+        hypothesis (d, n) against source s, and whether its homogeneous depth is
+        positive."""
+        v = self._view()
+        nn = np.ascontiguousarray(n, dtype=np.float64)
+        out = np.zeros(2)
+        ok = lib().orc_tap_coord(ctypes.byref(v), s, x, y, d, _dptr(nn), dx, dy, _dptr(out))
+        return out[0], out[1], bool(ok)
+
+    def draw(self, pixel, code, q):
+        """Draw counts per source of the Monte-Carlo view selection of one pixel
+        (random stream of (pixel, step code)) for the weights q."""
+        v, p = self._view(), self._p()
+        qq = np.ascontiguousarray(q, dtype=np.float64)
+        assert qq.shape == (self.S,)
+        cnt = np.zeros(self.S, dtype=np.int32)
+        lib().orc_draw(ctypes.byref(v), ctypes.byref(p), pixel, code, _dptr(qq),
+                       cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+        return cnt
+
     def photo_cost(self, x, y, d, n, s):
         v, p = self._view(), self._p()
         nn = np.ascontiguousarray(n, dtype=np.float64)
@@ -269,6 +303,29 @@ class OracleView:
         if do_filter:
             state["mask"] = self.filter(state, pass_)
         return state
+
+
+def step_code(pass_, it, color):
+    """Counter word of the random stream of one half-step."""
+    return (pass_ << 24) | (it << 1) | color
+
+
+def random_normal(params, u1, u2):
+    """Random hypothesis normal of the oracle for the uniforms (u1, u2)."""
+    p = to_orc_params(default_params(**params))
+    n = np.zeros(3)
+    lib().orc_random_normal(ctypes.byref(p), u1, u2, _dptr(n))
+    return n
+
+
+def perturb_normal(params, n, amax, u1, u2):
+    """Perturbed normal of the oracle: n turned by amax * u1 towards the
+    azimuth 2 pi u2 (n itself when the result leaves the cone)."""
+    p = to_orc_params(default_params(**params))
+    nn = np.ascontiguousarray(n, dtype=np.float64)
+    out = np.zeros(3)
+    lib().orc_perturb_normal(ctypes.byref(p), _dptr(nn), amax, u1, u2, _dptr(out))
+    return out
 
 
 def philox(ctr, key):

@@ -1,0 +1,371 @@
+"""Pins of the PatchMatch oracle's search (oracle/pm_oracle.c) on the CPU.
+
+The reference has no PatchMatch code (the adapted COLMAP binary is external),
+so the oracle's search is pinned by what the mathematics fixes, each check
+chosen so that a plausible mistake fails it:
+
+* plane transfer: on a source that is an exact warp of the reference under a
+  slanted plane, one pixel seeded with the true plane propagates it along its
+  row and column; every receiving pixel ends with the closed-form depth
+  d + a dx + b dy and zero cost (a flipped slope sign misses both);
+* window geometry: the source position of every window tap equals the
+  plane-induced homography of the reference's own debugger
+  (debuggers/sfm_debug_warping_subarea.py:45-57, restated here) for the world
+  plane the hypothesis describes, on satellite cameras 600 km away;
+* view selection: a matching source's selection probability rises to 1 and an
+  inverted source's falls to 0 half-step by half-step, and the Monte-Carlo
+  draws split between them in the proportion of their weights;
+* draws: over many pixels the counts follow q / sum(q) within a chi-square
+  bound, sum to num_samples, never pick a zero-weight source;
+* random normals: unit vectors inside the 80 degree cone, uniform over the
+  cap (cos(theta) uniform, Archimedes), uniform azimuth; perturbed normals lie
+  at exactly amax * u1 from the original with uniform azimuth around it.
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import pm_oracle as po
+
+
+def _view(ref, srcs, Ms, tri=None, params=None):
+    """An OracleView whose per-source maps M are given directly."""
+    ov = po.OracleView.__new__(po.OracleView)
+    p = po.default_params(depth_min=0.0, depth_max=100.0)
+    p.update(params or {})
+    ov.params = p
+    ov.ref = np.ascontiguousarray(ref, dtype=np.uint8)
+    ov.srcs = [np.ascontiguousarray(s, dtype=np.uint8) for s in srcs]
+    ov.H, ov.W = ov.ref.shape
+    ov.S = len(ov.srcs)
+    ov._M = np.ascontiguousarray(np.asarray(Ms, dtype=np.float64).reshape(ov.S, 12))
+    ov._Mb = ov._M.copy()
+    ov._tri = np.asarray(tri if tri is not None else [0.5] * ov.S, dtype=np.float64)
+    ov._g = np.tile([0.0, 0.0, 1.0], (ov.S, 1))
+    ov._J = np.eye(3).reshape(-1).copy()
+    ov._sw = np.array([s.shape[1] for s in ov.srcs], dtype=np.int32)
+    ov._sh = np.array([s.shape[0] for s in ov.srcs], dtype=np.int32)
+    ov._srcp = (ctypes.POINTER(ctypes.c_uint8) * ov.S)(
+        *[s.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)) for s in ov.srcs])
+    ov.set_source_depths(None)
+    ov.view_id = 3
+    ov.geo = {}
+    return ov
+
+
+IDENT = [[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 0]]
+
+
+def _texture(h, w, seed):
+    return np.random.default_rng(seed).integers(20, 230, (h, w)).astype(np.uint8)
+
+
+# ---------------------------------------------------------------------------
+# (a) plane transfer
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("slopes", [(1, 1), (1, 0), (0, 1)])
+def test_plane_transfer_reproduces_the_slanted_plane(slopes):
+    """Source column u = x + d (M adds the depth to the column).  With the
+    true plane d = d0 + a x + b y (integer a, b, d0) every tap lands on an
+    integer source column, and the source is built so that
+    src[v, (1+a) x + b v + d0] = ref[v, x]: the true plane has cost 0.
+    Pixel P = (12, 12) holds the true plane with stored cost 0, every other
+    pixel a wrong fronto-parallel plane with cost 2.  In the half-step of the
+    other colour, each pixel at odd distance <= 2 n_far - 1 from P on its row
+    or column receives P's plane transferred to its own position."""
+    a, b = slopes
+    d0 = 2
+    H = W = 24
+    ref = _texture(H, W, 11)
+    Ws = (1 + a) * (W - 1) + b * (H - 1) + d0 + 8
+    src = _texture(H, Ws, 12)
+    for y in range(H):
+        for x in range(W):
+            src[y, (1 + a) * x + b * y + d0] = ref[y, x]
+    M = [[1, 0, 0, 1], [0, 1, 0, 0], [0, 0, 1, 0]]
+    r, n_far = 2, 6
+    ov = _view(ref, [src], [M], params=dict(window_radius=r, n_far=n_far, depth_min=0.0,
+                                            depth_max=100.0))
+    st = ov.new_state()
+    ov.init(st, 0)
+    n_true = np.array([-a, -b, 1.0]) / math.sqrt(a * a + b * b + 1.0)
+    st["depth"][:] = 50.0
+    st["normal"][:] = np.tile([0.0, 0.0, 1.0], H * W)
+    st["cost"][:] = 2.0
+    px, py = 12, 12
+    ip = py * W + px
+    st["depth"][ip] = d0 + a * px + b * py
+    st["normal"][3 * ip:3 * ip + 3] = n_true
+    st["cost"][ip] = 0.0
+    # the true plane at P costs 0 (the construction is exact)
+    assert ov.photo_cost(px, py, d0 + a * px + b * py, n_true, 0) == pytest.approx(0.0, abs=1e-12)
+    color = 1 - ((px + py) & 1)
+    win, cand = ov.halfstep(st, 0, 0, color, diagnostics=True)
+    checked = 0
+    for k in range(n_far):
+        dist = 2 * k + 1
+        # direction index of the candidate that looks back towards P
+        for (x, y, direction) in ((px + dist, py, 0), (px - dist, py, 1),
+                                  (px, py + dist, 2), (px, py - dist, 3)):
+            if not (r <= x < W - r and r <= y < H - r):
+                continue
+            i = y * W + x
+            d_closed = d0 + a * x + b * y          # P's plane at (x, y)
+            assert st["depth"][i] == pytest.approx(d_closed, abs=1e-9), (x, y)
+            assert np.allclose(st["normal"][3 * i:3 * i + 3], n_true, atol=1e-12)
+            assert st["cost"][i] == pytest.approx(0.0, abs=1e-9)
+            assert cand[i, 1 + direction] == pytest.approx(0.0, abs=1e-9)
+            checked += 1
+    assert checked >= 12
+    # the flipped transfer d_P - a dx - b dy does not match the source
+    x, y = px + 3, py
+    n_flip = np.array([a, b, 1.0]) / math.sqrt(a * a + b * b + 1.0)
+    flipped = (d0 + a * px + b * py) - a * (x - px)
+    if a:
+        assert ov.photo_cost(x, y, flipped, n_flip, 0) > 0.05
+
+
+# ---------------------------------------------------------------------------
+# (e) window geometry against the plane-induced homography
+# ---------------------------------------------------------------------------
+def _ref_compute_homography(ref_P, src_P, plane_vec):
+    """Restatement of compute_homography (debuggers/sfm_debug_warping_subarea.py:45-57):
+    for the world plane n.X = c, x = P [X; 1] = S X with
+    S = P[:, :3] + P[:, 3] n^T / c, and H = S_ref S_src^-1 maps a source pixel
+    to the reference pixel."""
+    n = plane_vec[:3].reshape(3, 1)
+    c = plane_vec[3, 0]
+    ref_S = ref_P[:3, :3] + ref_P[:3, 3:4] @ n.T / c
+    src_S = src_P[:3, :3] + src_P[:3, 3:4] @ n.T / c
+    Hm = ref_S @ np.linalg.inv(src_S)
+    return Hm / np.max(np.abs(Hm))
+
+
+def _satellite_P4(off_nadir, azimuth, seed, width=512, height=512, z0=-20.0):
+    from vissatsatellitestereo_amd import synth
+    rng = np.random.default_rng(seed)
+    cam = synth.make_camera("v.png", width, height, off_nadir, azimuth, 1.0e6, 6.0e5, rng)
+    P = cam.P
+    X = np.array([0.0, 0.0, 0.0, 1.0])
+    depth_min = (P @ X)[2] - 50.0                # camera depth of the scene, minus a margin
+    last = depth_min * np.array([0.0, 0.0, 1.0, -z0])
+    return np.vstack([P, last])
+
+
+@pytest.mark.parametrize("geom", [(12.0, 0.0, 24.0, 120.0), (5.0, 45.0, 30.0, 200.0)])
+def test_tap_geometry_equals_plane_homography(geom):
+    """Every tap of every hypothesis lands where the homography of its world
+    plane sends the tap's reference pixel: the plane d = a u + b v + c0 in
+    (col, row, d) space is the world plane
+    (P4[3] - a P4[0] - b P4[1] - c0 P4[2]) . [X; 1] = 0."""
+    P4r = _satellite_P4(geom[0], geom[1], 1)
+    P4s = _satellite_P4(geom[2], geom[3], 2)
+    W = H = 512
+    geo = po.view_geometry(P4r, W, H, [P4s], (0.0, 200.0))
+    ov = _view(_texture(H, W, 1), [_texture(H, W, 2)], geo["M"])
+    rng = np.random.default_rng(3)
+    worst = 0.0
+    for _ in range(40):
+        x, y = int(rng.integers(20, W - 20)), int(rng.integers(20, H - 20))
+        d = float(rng.uniform(5.0, 195.0))
+        a, b = rng.uniform(-0.6, 0.6, 2) * (rng.random() < 0.8)
+        n = np.array([-a, -b, 1.0]) / math.sqrt(a * a + b * b + 1.0)
+        c0 = d - a * x - b * y
+        pi = P4r[3] - a * P4r[0] - b * P4r[1] - c0 * P4r[2]
+        plane_vec = np.append(pi[:3], -pi[3]).reshape(4, 1)
+        Hm = _ref_compute_homography(P4r[:3], P4s[:3], plane_vec)
+        Hinv = np.linalg.inv(Hm)
+        for dx, dy in ((0, 0), (-5, -5), (5, -3), (-4, 5), (3, 3)):
+            q = Hinv @ np.array([x + dx, y + dy, 1.0])
+            ex, ey = q[0] / q[2], q[1] / q[2]
+            gx, gy, ok = ov.tap_coord(0, x, y, d, n, dx, dy)
+            assert ok
+            worst = max(worst, abs(gx - ex), abs(gy - ey))
+    print("tap vs homography worst |dpx|", worst)
+    assert worst < 1e-4
+
+
+def test_tap_geometry_slope_sign_is_visible():
+    """The same check fails for the transposed or sign-flipped slopes, so the
+    comparison above resolves them (parallax of ~0.3 px per unit of d)."""
+    P4r = _satellite_P4(12.0, 0.0, 1)
+    P4s = _satellite_P4(24.0, 120.0, 2)
+    geo = po.view_geometry(P4r, 512, 512, [P4s], (0.0, 200.0))
+    ov = _view(_texture(512, 512, 1), [_texture(512, 512, 2)], geo["M"])
+    x, y, d, a, b = 200, 300, 80.0, 0.5, -0.3
+    c0 = d - a * x - b * y
+    pi = P4r[3] - a * P4r[0] - b * P4r[1] - c0 * P4r[2]
+    Hinv = np.linalg.inv(_ref_compute_homography(P4r[:3], P4s[:3], np.append(pi[:3], -pi[3]).reshape(4, 1)))
+    q = Hinv @ np.array([x + 5, y - 3, 1.0])
+    ex, ey = q[0] / q[2], q[1] / q[2]
+    n = np.array([-a, -b, 1.0]) / math.sqrt(a * a + b * b + 1.0)
+    gx, gy, _ = ov.tap_coord(0, x, y, d, n, 5, -3)
+    assert max(abs(gx - ex), abs(gy - ey)) < 1e-4
+    for (aa, bb) in ((-a, -b), (b, a), (a, -b), (-a, b)):
+        n = np.array([-aa, -bb, 1.0]) / math.sqrt(aa * aa + bb * bb + 1.0)
+        gx, gy, _ = ov.tap_coord(0, x, y, d, n, 5, -3)
+        assert max(abs(gx - ex), abs(gy - ey)) > 0.1, (aa, bb)
+
+
+def test_source_map_is_the_constant_depth_homography():
+    """A constant reparametrised depth d is a world plane (P4[3] - d P4[2]);
+    M_s sends every reference pixel at that depth where the plane's
+    homography sends it."""
+    P4r = _satellite_P4(8.0, 30.0, 5)
+    P4s = _satellite_P4(27.0, 250.0, 6)
+    geo = po.view_geometry(P4r, 512, 512, [P4s], (0.0, 200.0))
+    M = geo["M"][0].reshape(3, 4)
+    for d in (0.0, 37.5, 140.0):
+        pi = P4r[3] - d * P4r[2]
+        Hinv = np.linalg.inv(_ref_compute_homography(P4r[:3], P4s[:3], np.append(pi[:3], -pi[3]).reshape(4, 1)))
+        for (u, v) in ((0, 0), (511, 0), (256, 256), (17, 480)):
+            q = M @ np.array([u, v, 1.0, d])
+            e = Hinv @ np.array([u, v, 1.0])
+            assert q[0] / q[2] == pytest.approx(e[0] / e[2], abs=1e-4)
+            assert q[1] / q[2] == pytest.approx(e[1] / e[2], abs=1e-4)
+
+
+# ---------------------------------------------------------------------------
+# (b) view selection
+# ---------------------------------------------------------------------------
+def test_view_selection_converges_to_the_matching_source():
+    """Source 0 is an affine copy of the reference (cost 0 for every
+    hypothesis: M ignores depth), source 1 the inverted image (cost 2).  The
+    field is uniform, so every pixel of a colour carries the same selection
+    probability.  It rises monotonically to 1 for the match and falls to 0 for
+    the inverted source; each half-step's draws split between the two in the
+    proportion of the probabilities just computed (priors are 1 here)."""
+    H = W = 24
+    ref = _texture(H, W, 21)
+    srcs = [ref.astype(np.int32) + 7, 255 - ref.astype(np.int32)]
+    ov = _view(ref, srcs, [IDENT, IDENT], tri=[0.5, 0.5], params=dict(window_radius=2))
+    st = ov.new_state()
+    ov.init(st, 0)
+    HW = H * W
+    yy, xx = np.mgrid[0:H, 0:W]
+    a0_prev, a1_prev = 0.5, 0.5
+    n_updates = 0
+    for k in range(8):
+        it, color = k // 2, k % 2
+        ov.halfstep(st, 0, it, color)
+        act = ((xx + yy) % 2 == color).reshape(-1)
+        a0 = st["prob"][:HW][act]
+        a1 = st["prob"][HW:][act]
+        assert np.ptp(a0) < 1e-12 and np.ptp(a1) < 1e-12      # uniform field
+        a0, a1 = float(a0[0]), float(a1[0])
+        assert a0 > a0_prev and a1 < a1_prev
+        # aggregated cost = 2 * (draws of source 1) / num_samples
+        cnt1 = st["cost"][act] * 15 / 2.0
+        assert np.allclose(cnt1, np.round(cnt1), atol=1e-9)
+        p1 = a1 / (a0 + a1)
+        n_draws = 15 * act.sum()
+        mean, sd = n_draws * p1, math.sqrt(n_draws * p1 * (1 - p1))
+        assert abs(cnt1.sum() - mean) <= 5 * sd + 1, (k, cnt1.sum(), mean, sd)
+        a0_prev, a1_prev = a0, a1
+        n_updates += 1
+    assert a0_prev > 0.99 and a1_prev < 1e-4
+    assert n_updates == 8
+    # at the end the inverted source is (almost) never drawn: the last two
+    # half-steps expect ~0.2 of its draws over 576 pixels x 15 samples
+    assert np.count_nonzero(st["cost"] > 1e-12) <= 2
+
+
+# ---------------------------------------------------------------------------
+# (c) draw counts
+# ---------------------------------------------------------------------------
+def _draw_view(S):
+    img = _texture(8, 8, 0)
+    return _view(img, [img] * S, [IDENT] * S)
+
+
+def test_draw_counts_follow_weights():
+    q = np.array([0.5, 0.0, 0.2, 0.3, 0.0, 1.0])
+    ov = _draw_view(len(q))
+    N = 4000
+    code = po.step_code(0, 2, 1)
+    counts = np.array([ov.draw(i, code, q) for i in range(N)])
+    assert np.all(counts.sum(1) == 15)
+    assert np.all(counts[:, q == 0.0] == 0)
+    tot = counts.sum(0)
+    pos = q > 0
+    exp = 15 * N * q[pos] / q.sum()
+    chi2 = float(((tot[pos] - exp) ** 2 / exp).sum())
+    print("draw totals", tot, "expected", exp, "chi2", chi2)
+    assert chi2 < 30.0                        # 3 degrees of freedom, p ~ 2e-6
+    # per pixel, the count of a source of probability 1/2 is Binomial(15, 1/2)
+    ov2 = _draw_view(2)
+    c = np.array([ov2.draw(i, code, np.array([1.0, 1.0]))[0] for i in range(N)])
+    assert abs(c.mean() - 7.5) < 5 * math.sqrt(3.75 / N)
+    assert abs(c.var() - 3.75) < 0.3
+
+
+def test_draw_special_cases():
+    ov = _draw_view(4)
+    code = po.step_code(1, 0, 0)
+    # no positive weight: every source counts once
+    assert list(ov.draw(5, code, np.zeros(4))) == [1, 1, 1, 1]
+    # a single positive source takes every draw
+    assert list(ov.draw(5, code, np.array([0.0, 0.0, 2.5, 0.0]))) == [0, 0, 15, 0]
+    # different pixels and half-steps draw from different streams
+    q = np.ones(4)
+    rows = {tuple(ov.draw(i, code, q)) for i in range(50)}
+    assert len(rows) > 20
+
+
+# ---------------------------------------------------------------------------
+# (d) random and perturbed normals
+# ---------------------------------------------------------------------------
+def _ks_uniform(x):
+    """Kolmogorov-Smirnov distance of samples x (in [0,1]) to U(0,1)."""
+    x = np.sort(x)
+    n = len(x)
+    i = np.arange(1, n + 1)
+    return max(np.max(i / n - x), np.max(x - (i - 1) / n))
+
+
+def test_random_normals_uniform_on_the_cap():
+    rng = np.random.default_rng(8)
+    N = 4000
+    u = rng.random((N, 2))
+    n = np.array([po.random_normal({}, a, b) for a, b in u])
+    assert np.allclose(np.linalg.norm(n, axis=1), 1.0, atol=1e-12)
+    cmax = math.cos(math.radians(80.0))
+    assert np.all(n[:, 2] >= cmax - 1e-12)
+    # uniform over the spherical cap <=> cos(theta) uniform on [cos 80, 1]
+    assert _ks_uniform((n[:, 2] - cmax) / (1.0 - cmax)) < 1.63 / math.sqrt(N)
+    phi = np.mod(np.arctan2(n[:, 1], n[:, 0]), 2 * math.pi) / (2 * math.pi)
+    assert _ks_uniform(phi) < 1.63 / math.sqrt(N)
+    # both limits of the cap are reached
+    assert po.random_normal({}, 0.0, 0.3)[2] == pytest.approx(1.0)
+    assert po.random_normal({}, 1.0, 0.3)[2] == pytest.approx(cmax, abs=1e-12)
+
+
+def test_perturbed_normals_turn_by_the_drawn_angle():
+    rng = np.random.default_rng(9)
+    N = 3000
+    amax = 0.3
+    base = po.random_normal({}, 0.3, 0.6)     # ~43 degrees off the axis
+    az = []
+    for _ in range(N):
+        u1, u2 = rng.random(2)
+        m = po.perturb_normal({}, base, amax, u1, u2)
+        assert np.linalg.norm(m) == pytest.approx(1.0, abs=1e-12)
+        ang = math.acos(np.clip(m @ base, -1.0, 1.0))
+        assert ang == pytest.approx(amax * u1, abs=1e-7)
+        # azimuth of m around base, measured in a fixed frame of base
+        e1 = np.cross(base, [0.0, 0.0, 1.0])
+        e1 /= np.linalg.norm(e1)
+        e2 = np.cross(base, e1)
+        t = m - (m @ base) * base
+        if np.linalg.norm(t) > 1e-6:
+            az.append(np.mod(math.atan2(t @ e2, t @ e1), 2 * math.pi) / (2 * math.pi))
+    assert _ks_uniform(np.array(az)) < 1.63 / math.sqrt(len(az))
+    # a perturbation that would leave the cone keeps the normal
+    edge = po.random_normal({}, 1.0, 0.0)      # on the 80 degree rim
+    out = [po.perturb_normal({}, edge, 0.5, 1.0, u2) for u2 in np.linspace(0, 1, 16, endpoint=False)]
+    kept = [np.array_equal(o, edge) for o in out]
+    assert any(kept) and not all(kept)
+    cmax = math.cos(math.radians(80.0))
+    assert all(o[2] >= cmax - 1e-12 for o in out)

@@ -206,25 +206,61 @@ int vsm_get_depth_map(vsm_ctx *ctx, int32_t image_id, int32_t kind, float *depth
 /* Solve n reference views in one batched launch sequence and block until
  * done.  pass 0 = photometric (run_photometric_mvs,
  * colmap_mvs_commands.py:39-55), pass 1 = geometric consistency + filter
- * (run_consistency_check, colmap_mvs_commands.py:58-82).  With
- * use_graph != 0 the half-step sequence is captured once in a hipGraph and
- * replayed. */
+ * (run_consistency_check, colmap_mvs_commands.py:58-82).  The geometric
+ * pass starts from the request's init_depth/init_normal when given, else
+ * from the reference image's photometric depth and normal maps on the device
+ * (a photometric solve keeps them; vsm_set_depth_map/vsm_set_normal_map set
+ * them from files).  With use_graph != 0 the half-step sequence is captured
+ * once in a hipGraph and replayed. */
 int vsm_solve(vsm_ctx *ctx, const vsm_view_req *reqs, int32_t n, int32_t pass,
               vsm_view_out *outs, int32_t use_graph);
 
-/* Serving: non-blocking admission.  The request is copied; it is admitted
- * when its HBM footprint fits the budget (else it waits in FIFO order), joins
- * the running batch at the next half-step boundary, and runs the photometric
- * pass (num_iterations) followed by the filter when params.filter is set
- * (photometric consistency only: NCC and triangulation angle).  A request's
- * result does not depend on which other requests share its launches.
- * vsm_poll copies the result into out when done (*done = 1), else *done = 0.
- * A finished request's depth also becomes its image's photometric depth map
- * on the device (vsm_get_depth_map kind 0, vsm_fuse_dsm depth_kind 0), as
- * after vsm_solve pass 0.
- * vsm_serve_step admits what fits and advances every admitted request by one
- * half-step (one batched launch). */
+/* vsm_solve in stages (the same launches and results): vsm_batch_begin
+ * prepares the n views and sets their initial hypotheses (*batch receives a
+ * handle); vsm_batch_step runs up to n_halfsteps more red-black half-steps
+ * over all of them, one persistent launch per half-step (with use_graph the
+ * sequence of that length is captured into a hipGraph on first use and
+ * replayed on later calls), and sets *remaining (may be NULL) to the
+ * half-steps left of the pass's 2 * num_iterations; vsm_batch_end runs the
+ * filter (if the parameters at begin ask for it), writes the outputs as
+ * vsm_solve does and releases the batch -- VSM_EINVAL while half-steps
+ * remain, unless discard != 0, which releases it without outputs.  The
+ * parameters in force at begin hold for the whole batch.  Lets a caller time
+ * or interleave the half-steps of several solves; the reference runs the
+ * whole pass inside one subprocess (colmap_mvs_commands.py:39-82). */
+int vsm_batch_begin(vsm_ctx *ctx, const vsm_view_req *reqs, int32_t n, int32_t pass,
+                    int32_t use_graph, int32_t *batch);
+int vsm_batch_step(vsm_ctx *ctx, int32_t batch, int32_t n_halfsteps, int32_t *remaining);
+int vsm_batch_end(vsm_ctx *ctx, int32_t batch, vsm_view_out *outs, int32_t discard);
+
+/* Serving: non-blocking admission of per-view solves (the request is copied).
+ * vsm_submit_pass queues a solve of pass 0 (photometric,
+ * colmap_mvs_commands.py:39-55) or 1 (geometric consistency,
+ * colmap_mvs_commands.py:58-82), with the filter on (1), off (0) or as
+ * params.filter (-1); vsm_submit is vsm_submit_pass(pass 0, filter -1).
+ * Results are those of solving the requests one at a time in submission
+ * order: a geometric request starts once every earlier photometric request of
+ * its reference or its sources has finished (and fails with VSM_EINVAL when
+ * the reference has no photometric result or a source no photometric depth
+ * map and no such request is pending); a photometric request waits for every
+ * earlier request that reads or writes its image's photometric maps.  A
+ * request that can start is admitted when its HBM footprint fits the budget
+ * (ready requests are admitted in submission order; one that cannot fit even
+ * with nothing else active fails with VSM_ENOMEM), joins the running batch at
+ * the next half-step boundary and shares its launches with the others; its
+ * result does not depend on which requests share them.  Parameters in force
+ * at each vsm_serve_step apply to every active request: change them only
+ * between requests.  vsm_poll copies the result into out when done
+ * (*done = 1), else *done = 0.  A finished request's depth and normals also
+ * become its image's device maps of that pass (vsm_get_depth_map,
+ * vsm_fuse_dsm; a geometric pass starts from the photometric ones), as after
+ * vsm_solve.
+ * vsm_serve_step admits what can start and advances every admitted request
+ * by one half-step (one batched launch).  An image used by a queued or
+ * active request cannot be removed or replaced. */
 int vsm_submit(vsm_ctx *ctx, const vsm_view_req *req, uint64_t *ticket);
+int vsm_submit_pass(vsm_ctx *ctx, const vsm_view_req *req, int32_t pass, int32_t filter,
+                    uint64_t *ticket);
 int vsm_serve_step(vsm_ctx *ctx, int32_t *n_active, int32_t *n_queued);
 int vsm_poll(vsm_ctx *ctx, uint64_t ticket, vsm_view_out *out, int32_t *done);
 
@@ -310,6 +346,12 @@ int vsm_fuse_points(vsm_ctx *ctx, const int32_t *image_ids, int32_t n, int32_t d
  * (cv2.medianBlur(dsm, 3) at produce_dsm.py:58 and aggregate_2p5d.py:81);
  * NaN excluded from each window, all-NaN -> NaN. */
 int vsm_median_blur3(vsm_ctx *ctx, const float *in, int32_t e_size, int32_t n_size, float *out);
+
+/* Change the context's HBM budget (0 = unlimited); freed blocks the context
+ * caches beyond it are released.  Later allocations and serving admissions
+ * are checked against the new value; nothing already allocated is moved.
+ * The reference has no budget (the colmap binary takes what it needs). */
+int vsm_set_hbm_budget(vsm_ctx *ctx, uint64_t hbm_budget_bytes);
 
 int vsm_get_stats(vsm_ctx *ctx, vsm_stats *out);
 int vsm_reset_stats(vsm_ctx *ctx);

@@ -297,6 +297,36 @@ def test_batched_equals_single(tiny):
         assert np.array_equal(g["depth"], b["depth"])
 
 
+def test_staged_batch_equals_solve(tiny):
+    """vsm_batch_begin / _step / _end (one half-step per call, with and
+    without the hipGraph, reused across calls) give exactly vsm_solve's
+    results for both passes; a discarded batch leaves the context usable."""
+    p = _params(window_radius=4, num_iterations=3)
+    ctx, _, _ = _setup(tiny, p)
+    names = tiny["names"]
+    views = [dict(ref=_id(tiny, r), srcs=[_id(tiny, s) for s in names if s != r],
+                  depth_range=tiny["dr"][r]) for r in names]
+    want = ("depth", "normal", "cost", "consistency")
+    ref0 = ctx.solve(views, 0, want=want, use_graph=True)
+    ctx.set_params(geom_consistency=1, filter=1)
+    ref1 = ctx.solve(views, 1, want=want, use_graph=True)
+    ctx.set_params(geom_consistency=0, filter=0)
+    junk = ctx.batch(views, 0, use_graph=True)
+    junk.step(2)
+    junk.discard()
+    for graph in (True, False):
+        for pass_, ref in ((0, ref0), (1, ref1)):
+            ctx.set_params(geom_consistency=pass_, filter=pass_)
+            b = ctx.batch(views, pass_, use_graph=graph)
+            left = [b.step(1) for _ in range(6)]
+            assert left == [5, 4, 3, 2, 1, 0]
+            assert b.step(1) == 0
+            res = b.end(want)
+            for i in range(len(views)):
+                for k in want:
+                    assert np.array_equal(res[i][k], ref[i][k]), (graph, pass_, i, k)
+
+
 def test_lockstep_partial_tiles(ragged):
     """Reference 150 x 100 (partial tiles on two edges), window step 2."""
     assert ragged["imgs"][ragged["names"][0]].shape == (100, 150)

@@ -219,3 +219,67 @@ def test_consistency_graph_records_round_trip(tmp_path):
     assert raw.size == sum(3 + len(v) for v in exp.values())
     cg.write(path, np.zeros((3, 3), np.uint32), src)
     assert cg.read(path) == {}
+
+
+ANCHOR_PROBE = r"""
+#include <cstdio>
+#include <cmath>
+#include <cstdint>
+#include "vsm_device.h"
+// the gathers' byte offset: signed 24-bit operands, low 32 bits of the product
+static int32_t mul24(int32_t a, int32_t b)
+{
+    const int64_t sa = (int64_t)(int32_t)((uint32_t)a << 8) >> 8;
+    const int64_t sb = (int64_t)(int32_t)((uint32_t)b << 8) >> 8;
+    return (int32_t)(uint32_t)(uint64_t)(sa * sb);
+}
+int main()
+{
+    const double big[] = {0.0, 1.0, -1.0, 3.5e6, -3.5e6, 8.4e6, 1.0e7, -1.0e7, 1.0e12, -1.0e12,
+                          INFINITY, -INFINITY, NAN};
+    const int sizes[][2] = {{2, 2}, {256, 256}, {4096, 4096}, {32768, 32768}, {32768, 2}};
+    long bad = 0, n = 0;
+    for (auto &sz : sizes)
+        for (double qx : big)
+            for (double qy : big)
+                for (double qz : {1e-300, 1.0, 0.0, -1.0, 1e300}) {
+                    const int W = sz[0], H = sz[1];
+                    int ax, ay;
+                    vsm::source_anchor(qx, qy, qz, W, H, ax, ay);
+                    if (ax < 0 || ax >= W || ay < 0 || ay >= H) { ++bad; continue; }
+                    // every tap clamped to the image box, relative to the anchor
+                    const int xs[] = {-ax, 0, W - 1 - ax}, ys[] = {-ay, 0, H - 1 - ay};
+                    const int W4 = 4 * W, base4 = (int)(uint32_t)(4u * (uint32_t)(ay * W + ax));
+                    for (int ix : xs)
+                        for (int iy : ys) {
+                            const uint32_t off = (uint32_t)(mul24(iy, W4) + ((ix << 2) + base4));
+                            const int64_t want = 4ll * ((int64_t)(ay + iy) * W + ax + ix);
+                            ++n;
+                            if ((int64_t)off != want) ++bad;
+                        }
+                }
+    printf("%ld %ld\n", n, bad);
+    return bad != 0;
+}
+"""
+
+
+def test_source_anchor_keeps_gather_offsets_in_range(tmp_path):
+    """The per-tile source anchor stays inside the image for projections far
+    outside it, behind the camera, at infinity or NaN, so the 24-bit offset
+    arithmetic of the window gathers gives the exact texel offset for every
+    tap clamped to the image box (images up to 32768 px a side)."""
+    import shutil
+    import subprocess
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not found")
+    src = tmp_path / "anchor.cpp"
+    src.write_text(ANCHOR_PROBE)
+    exe = tmp_path / "anchor"
+    csrc = os.path.join(ROOT, "vissatsatellitestereo_amd", "csrc")
+    subprocess.run([hipcc, "-x", "hip", "--offload-arch=gfx950", "-O1", "-I", csrc, "-I",
+                    os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    n, bad = (int(v) for v in out.stdout.split())
+    assert n > 10000 and bad == 0 and out.returncode == 0, out.stdout

@@ -118,7 +118,13 @@ SIGNATURES = [
     ("vsm_get_depth_map", _c.c_int, [_c.c_void_p, _c.c_int32, _c.c_int32, P(_c.c_float), _c.c_int32,
                                      _c.c_int32]),
     ("vsm_solve", _c.c_int, [_c.c_void_p, P(ViewReq), _c.c_int32, _c.c_int32, P(ViewOut), _c.c_int32]),
+    ("vsm_batch_begin", _c.c_int, [_c.c_void_p, P(ViewReq), _c.c_int32, _c.c_int32, _c.c_int32,
+                                   P(_c.c_int32)]),
+    ("vsm_batch_step", _c.c_int, [_c.c_void_p, _c.c_int32, _c.c_int32, P(_c.c_int32)]),
+    ("vsm_batch_end", _c.c_int, [_c.c_void_p, _c.c_int32, P(ViewOut), _c.c_int32]),
+    ("vsm_set_hbm_budget", _c.c_int, [_c.c_void_p, _c.c_uint64]),
     ("vsm_submit", _c.c_int, [_c.c_void_p, P(ViewReq), P(_c.c_uint64)]),
+    ("vsm_submit_pass", _c.c_int, [_c.c_void_p, P(ViewReq), _c.c_int32, _c.c_int32, P(_c.c_uint64)]),
     ("vsm_serve_step", _c.c_int, [_c.c_void_p, P(_c.c_int32), P(_c.c_int32)]),
     ("vsm_poll", _c.c_int, [_c.c_void_p, _c.c_uint64, P(ViewOut), P(_c.c_int32)]),
     ("vsm_eval_cost", _c.c_int, [_c.c_void_p, P(ViewReq), _c.c_int32, P(_c.c_float), P(_c.c_float),

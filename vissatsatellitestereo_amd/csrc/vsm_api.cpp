@@ -39,7 +39,6 @@ struct ImageRec {
     float *normal[2] = {nullptr, nullptr};  // camera-frame normals that go with depth[]
     float4 *depth4 = nullptr;               // photometric depth, 2x2 neighbourhood per texel
     uint64_t depth_ver = 0, depth4_ver = 0; // depth4 is current when the versions agree
-    float4 *hyp_photo = nullptr;            // photometric hypotheses (geometric init)
 };
 
 struct ViewState {
@@ -47,21 +46,40 @@ struct ViewState {
     float4 *hyp = nullptr;
     float *cost = nullptr;
     float *prob = nullptr;
-    bool own_hyp = true;
+};
+
+// A staged batched solve (vsm_batch_begin / _step / _end): the views' state,
+// the device view descriptors and tile list, and the instantiated hipGraphs
+// of its half-step sequences, kept for reuse across step calls.
+struct Batch {
+    int pass = 0;
+    bool use_graph = false;
+    vsm_params params;                      // snapshot at begin: every step and the filter use it
+    std::vector<ViewState> views;
+    std::vector<int> ref_ids;
+    ViewDesc *dviews = nullptr;
+    Tile *dtiles = nullptr;
+    int ntiles = 0, maxs = 1;
+    int steps_done = 0, nsteps = 0;
+    std::vector<std::pair<int, hipGraphExec_t>> graphs;   // (half-steps per replay, executable)
 };
 
 struct ServeReq {
     uint64_t ticket = 0;
     int ref = 0;
+    int pass = 0;                           // 0 photometric, 1 geometric consistency
+    int filter = 0;
     std::vector<int> srcs;
     float dmin = 0.f, dmax = 0.f;
     // device state while active
     ViewState vs;
-    int slot = -1;
     int host_step = 0;
-    size_t footprint = 0;
-    // results (device) when done
+    // results (device) when done: depth, normal (3), cost, consistency mask
     float *out_depth = nullptr, *out_normal = nullptr, *out_cost = nullptr;
+    uint32_t *out_mask = nullptr;
+    // the image's depth and normal maps of this pass when it has none yet
+    // (allocated at admission, handed to the image at retirement)
+    float *keep_depth = nullptr, *keep_normal = nullptr;
     bool done = false;
     int error = 0;
     std::string errmsg;
@@ -102,7 +120,10 @@ struct vsm_ctx {
     std::map<uint64_t, std::unique_ptr<ServeReq>> reqs;
     ViewDesc *serve_views = nullptr;
     Tile *serve_tiles = nullptr;
-    int serve_cap_views = 0, serve_cap_tiles = 0;
+    int serve_cap_tiles = 0;
+    // staged batched solves
+    std::map<int, std::unique_ptr<Batch>> batches;
+    int next_batch = 1;
 };
 
 namespace {
@@ -516,7 +537,7 @@ int prepare_view(vsm_ctx *c, const vsm_view_req &r, int pass, uint32_t view_id, 
 
 void release_view(vsm_ctx *c, ViewState &vs)
 {
-    if (vs.own_hyp) dfree(c, vs.hyp);
+    dfree(c, vs.hyp);
     dfree(c, vs.cost);
     dfree(c, vs.prob);
     vs.hyp = nullptr; vs.cost = nullptr; vs.prob = nullptr;
@@ -568,141 +589,16 @@ int record_halfstep(vsm_ctx *c, int maxs, const ViewDesc *dviews, const Tile *dt
     return VSM_OK;
 }
 
-int read_counters(vsm_ctx *c)
+int read_counters(vsm_ctx *c, const PMConst &pc)
 {
     Counters h;
     HIPCHK(c, hipMemcpyAsync(&h, c->d_counters, sizeof h, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, launch_fill_u64(reinterpret_cast<unsigned long long *>(c->d_counters), 4, 0ull, c->stream));
-    const PMConst pc = make_const(c->params);
     c->acc_ncc += (int64_t)h.ncc_evals;
     c->acc_geo += (int64_t)h.geo_evals;
     c->acc_taps += (int64_t)h.ncc_evals * pc.ntaps * pc.ntaps;
     return VSM_OK;
-}
-
-// Batched solve core: views already prepared; runs init/iterations/filter.
-int run_views(vsm_ctx *c, std::vector<ViewState> &views, const std::vector<const vsm_view_req *> &reqs,
-              int pass, uint32_t *d_masks, const std::vector<size_t> &mask_off, int use_graph)
-{
-    const int n = (int)views.size();
-    const PMConst pc = make_const(c->params);
-    int maxs = 1;
-    std::vector<Tile> tiles;
-    for (int v = 0; v < n; ++v) {
-        maxs = std::max(maxs, views[v].desc.n_src);
-        tiles_for(views[v].desc, v, tiles);
-    }
-    ViewDesc *dviews = nullptr;
-    Tile *dtiles = nullptr;
-    int rc = dalloc(c, &dviews, n);
-    if (!rc) rc = dalloc(c, &dtiles, tiles.size());
-    if (!rc) rc = ensure_c0(c, maxs);     // before any capture
-    if (rc) { dfree(c, dviews); dfree(c, dtiles); return rc; }
-    std::vector<ViewDesc> hv(n);
-    for (int v = 0; v < n; ++v) hv[v] = views[v].desc;
-    auto cleanup = [&]() { dfree(c, dviews); dfree(c, dtiles); };
-#define RUNCHK(expr)                                                                            \
-    do {                                                                                        \
-        hipError_t e_ = (expr);                                                                 \
-        if (e_ != hipSuccess) {                                                                 \
-            cleanup();                                                                          \
-            return fail(c, VSM_EDEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));        \
-        }                                                                                       \
-    } while (0)
-    RUNCHK(hipMemcpyAsync(dviews, hv.data(), n * sizeof(ViewDesc), hipMemcpyHostToDevice, c->stream));
-    RUNCHK(hipMemcpyAsync(dtiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice,
-                          c->stream));
-    const int ntiles = (int)tiles.size();
-    // initial hypotheses
-    if (pass == 0) {
-        RUNCHK(launch_init(dviews, dtiles, ntiles, pc, 1, c->stream));
-    } else {
-        for (int v = 0; v < n; ++v) {
-            const vsm_view_req &r = *reqs[v];
-            const ImageRec &ref = c->images[r.ref_id];
-            const size_t hw = (size_t)ref.W * ref.H;
-            if (ref.hyp_photo) {
-                RUNCHK(hipMemcpyAsync(views[v].hyp, ref.hyp_photo, hw * sizeof(float4),
-                                      hipMemcpyDeviceToDevice, c->stream));
-            } else if (r.init_depth && r.init_normal) {
-                float *tmp = nullptr;
-                if ((rc = dalloc(c, &tmp, hw * 4))) { cleanup(); return rc; }
-                RUNCHK(hipMemcpyAsync(tmp, r.init_depth, hw * 4, hipMemcpyHostToDevice, c->stream));
-                RUNCHK(hipMemcpyAsync(tmp + hw, r.init_normal, hw * 12, hipMemcpyHostToDevice, c->stream));
-                RUNCHK(launch_load_hyp(dviews, v, (int)hw, tmp, tmp + hw, pc.cos_max_normal, c->stream));
-                RUNCHK(hipStreamSynchronize(c->stream));
-                dfree(c, tmp);
-            } else {
-                cleanup();
-                return fail(c, VSM_EINVAL,
-                            "geometric pass of image %d needs its photometric result (solve pass 0 "
-                            "first or pass init_depth/init_normal)", r.ref_id);
-            }
-        }
-        RUNCHK(launch_init(dviews, dtiles, ntiles, pc, 0, c->stream));
-    }
-    const int nsteps = 2 * c->params.num_iterations;
-    if (use_graph && nsteps > 0) {
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        RUNCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        hipError_t ce = hipSuccess;
-        for (int s = 0; s < nsteps && ce == hipSuccess; ++s) {
-            ce = launch_halfstep(maxs, dviews, dtiles, ntiles, pc, c->d_counters, c->d_work,
-                                 grid_for(c, maxs), c->d_c0, c->stream);
-            if (ce == hipSuccess) ce = launch_advance(dviews, n, c->stream);
-        }
-        hipError_t ee = hipStreamEndCapture(c->stream, &graph);
-        if (ce != hipSuccess || ee != hipSuccess) {
-            if (graph) (void)hipGraphDestroy(graph);
-            cleanup();
-            return fail(c, VSM_EDEVICE, "graph capture failed: %s",
-                        hipGetErrorString(ce != hipSuccess ? ce : ee));
-        }
-        RUNCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-        if (c->timing) RUNCHK(hipEventRecord(c->ev0, c->stream));
-        RUNCHK(hipGraphLaunch(exec, c->stream));
-        if (c->timing) {
-            RUNCHK(hipEventRecord(c->ev1, c->stream));
-            RUNCHK(hipEventSynchronize(c->ev1));
-            float ms = 0.f;
-            RUNCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-            c->halfstep_ms += ms;
-        }
-        c->halfstep_launches += nsteps;
-        RUNCHK(hipStreamSynchronize(c->stream));
-        (void)hipGraphExecDestroy(exec);
-        (void)hipGraphDestroy(graph);
-    } else {
-        for (int s = 0; s < nsteps; ++s) {
-            if ((rc = record_halfstep(c, maxs, dviews, dtiles, ntiles, pc, c->timing))) {
-                cleanup();
-                return rc;
-            }
-            RUNCHK(launch_advance(dviews, n, c->stream));
-        }
-    }
-    // host tile lists stay alive until the stream is synchronised below
-    std::vector<std::vector<Tile>> filter_tiles;
-    if (c->params.filter) {
-        filter_tiles.resize(n);
-        for (int v = 0; v < n; ++v) {
-            // per-view launch so the mask buffer offsets stay simple
-            std::vector<Tile> &vt = filter_tiles[v];
-            tiles_for(views[v].desc, 0, vt);
-            Tile *dt = nullptr;
-            if ((rc = dalloc(c, &dt, vt.size()))) { cleanup(); return rc; }
-            RUNCHK(hipMemcpyAsync(dt, vt.data(), vt.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream));
-            RUNCHK(launch_filter(maxs, dviews + v, dt, (int)vt.size(), pc,
-                                 d_masks ? d_masks + mask_off[v] : nullptr, c->stream));
-            dfree(c, dt);                      // back to the cache, stream-ordered
-        }
-    }
-    RUNCHK(hipStreamSynchronize(c->stream));
-    cleanup();
-#undef RUNCHK
-    return read_counters(c);
 }
 
 int finalize_outputs(vsm_ctx *c, ViewState &vs, int pass, int ref_id, vsm_view_out *out,
@@ -740,13 +636,212 @@ int finalize_outputs(vsm_ctx *c, ViewState &vs, int pass, int ref_id, vsm_view_o
     // any later use of them on the stream, and the caller synchronises
     dfree(c, dv);
     dfree(c, tmp);
-    if (pass == 0) {
-        // keep the photometric hypotheses for a later geometric pass
-        if (ref.hyp_photo) dfree(c, ref.hyp_photo);
-        ref.hyp_photo = vs.hyp;
-        vs.own_hyp = false;
-    }
     return VSM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Staged batched solve.  A batch holds n prepared views; each step runs
+// half-steps over all of them with one persistent launch per half-step
+// (optionally replayed from a captured hipGraph, kept for later steps of the
+// same length); the end runs the filter and writes the outputs.  vsm_solve is
+// begin + every half-step + end.
+// ---------------------------------------------------------------------------
+void batch_release(vsm_ctx *c, Batch &b)
+{
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &g : b.graphs) (void)hipGraphExecDestroy(g.second);
+    b.graphs.clear();
+    for (auto &v : b.views) release_view(c, v);
+    b.views.clear();
+    dfree(c, b.dviews);
+    dfree(c, b.dtiles);
+    b.dviews = nullptr;
+    b.dtiles = nullptr;
+}
+
+// Prepare the views, upload their descriptors and the concatenated tile list,
+// and set the initial hypotheses: random for the photometric pass; for the
+// geometric pass the depth/normal maps the request supplies, else the
+// image's photometric depth and normal maps on the device (kept by its
+// photometric solve or set from files).
+int batch_begin(vsm_ctx *c, const vsm_view_req *reqs, int n, int pass, int use_graph, Batch &b)
+{
+    b.pass = pass;
+    b.use_graph = use_graph != 0;
+    b.params = c->params;
+    b.views.resize(n);
+    int rc = VSM_OK, prepared = 0;
+    for (int v = 0; v < n && !rc; ++v) {
+        rc = prepare_view(c, reqs[v], pass, (uint32_t)reqs[v].ref_id, b.views[v]);
+        if (!rc) ++prepared;
+    }
+    if (rc) {
+        b.views.resize(prepared);
+        batch_release(c, b);
+        return rc;
+    }
+    std::vector<Tile> tiles;
+    std::vector<ViewDesc> hv(n);
+    for (int v = 0; v < n; ++v) {
+        b.maxs = std::max(b.maxs, b.views[v].desc.n_src);
+        tiles_for(b.views[v].desc, v, tiles);
+        b.ref_ids.push_back(reqs[v].ref_id);
+        hv[v] = b.views[v].desc;
+    }
+    b.ntiles = (int)tiles.size();
+    b.nsteps = 2 * b.params.num_iterations;
+    rc = dalloc(c, &b.dviews, n);
+    if (!rc) rc = dalloc(c, &b.dtiles, tiles.size());
+    if (!rc) rc = ensure_c0(c, b.maxs);     // before any capture
+    if (rc) {
+        batch_release(c, b);
+        return rc;
+    }
+#define BEGINCHK(expr)                                                                          \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            batch_release(c, b);                                                                \
+            return fail(c, VSM_EDEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));        \
+        }                                                                                       \
+    } while (0)
+    const PMConst pc = make_const(b.params);
+    BEGINCHK(hipMemcpyAsync(b.dviews, hv.data(), n * sizeof(ViewDesc), hipMemcpyHostToDevice, c->stream));
+    BEGINCHK(hipMemcpyAsync(b.dtiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice,
+                            c->stream));
+    if (pass == 0) {
+        BEGINCHK(launch_init(b.dviews, b.dtiles, b.ntiles, pc, 1, c->stream));
+    } else {
+        for (int v = 0; v < n; ++v) {
+            const vsm_view_req &r = reqs[v];
+            const ImageRec &ref = c->images[r.ref_id];
+            const size_t hw = (size_t)ref.W * ref.H;
+            if (r.init_depth && r.init_normal) {
+                float *tmp = nullptr;
+                if ((rc = dalloc(c, &tmp, hw * 4))) {
+                    batch_release(c, b);
+                    return rc;
+                }
+                BEGINCHK(hipMemcpyAsync(tmp, r.init_depth, hw * 4, hipMemcpyHostToDevice, c->stream));
+                BEGINCHK(hipMemcpyAsync(tmp + hw, r.init_normal, hw * 12, hipMemcpyHostToDevice, c->stream));
+                BEGINCHK(launch_load_hyp(b.dviews, v, (int)hw, tmp, tmp + hw, pc.cos_max_normal, c->stream));
+                BEGINCHK(hipStreamSynchronize(c->stream));
+                dfree(c, tmp);
+            } else if (ref.depth[0] && ref.normal[0]) {
+                BEGINCHK(launch_load_hyp(b.dviews, v, (int)hw, ref.depth[0], ref.normal[0],
+                                         pc.cos_max_normal, c->stream));
+            } else {
+                batch_release(c, b);
+                return fail(c, VSM_EINVAL,
+                            "geometric pass of image %d needs its photometric depth and normal maps "
+                            "(solve pass 0 first, set them, or pass init_depth/init_normal)", r.ref_id);
+            }
+        }
+        BEGINCHK(launch_init(b.dviews, b.dtiles, b.ntiles, pc, 0, c->stream));
+    }
+    BEGINCHK(hipStreamSynchronize(c->stream));
+#undef BEGINCHK
+    return VSM_OK;
+}
+
+// Run up to k more half-steps (fewer when the batch has fewer left).
+int batch_step(vsm_ctx *c, Batch &b, int k)
+{
+    const int m = std::min(k, b.nsteps - b.steps_done);
+    if (m <= 0) return VSM_OK;
+    const PMConst pc = make_const(b.params);
+    if (b.use_graph) {
+        hipGraphExec_t exec = nullptr;
+        for (auto &g : b.graphs)
+            if (g.first == m) exec = g.second;
+        if (!exec) {
+            // [zero the tile cursors, half-step, advance the device step
+            // counters] x m, captured once per length and replayed
+            hipGraph_t graph = nullptr;
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t ce = hipSuccess;
+            for (int s = 0; s < m && ce == hipSuccess; ++s) {
+                ce = launch_halfstep(b.maxs, b.dviews, b.dtiles, b.ntiles, pc, c->d_counters, c->d_work,
+                                     grid_for(c, b.maxs), c->d_c0, c->stream);
+                if (ce == hipSuccess) ce = launch_advance(b.dviews, (int)b.views.size(), c->stream);
+            }
+            const hipError_t ee = hipStreamEndCapture(c->stream, &graph);
+            if (ce != hipSuccess || ee != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return fail(c, VSM_EDEVICE, "graph capture failed: %s",
+                            hipGetErrorString(ce != hipSuccess ? ce : ee));
+            }
+            const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess)
+                return fail(c, VSM_EDEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(ie));
+            b.graphs.emplace_back(m, exec);
+        }
+        if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+        HIPCHK(c, hipGraphLaunch(exec, c->stream));
+        if (c->timing) {
+            HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+            HIPCHK(c, hipEventSynchronize(c->ev1));
+            float ms = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            c->halfstep_ms += ms;
+        }
+        c->halfstep_launches += m;
+    } else {
+        for (int s = 0; s < m; ++s) {
+            const int rc = record_halfstep(c, b.maxs, b.dviews, b.dtiles, b.ntiles, pc, c->timing);
+            if (rc) return rc;
+            HIPCHK(c, launch_advance(b.dviews, (int)b.views.size(), c->stream));
+        }
+    }
+    b.steps_done += m;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return read_counters(c, pc);
+}
+
+// Filter (when the batch's parameters ask for it) and outputs of every view.
+int batch_finish(vsm_ctx *c, Batch &b, vsm_view_out *outs)
+{
+    const int n = (int)b.views.size();
+    const PMConst pc = make_const(b.params);
+    uint32_t *d_masks = nullptr;
+    std::vector<size_t> mask_off(n, 0);
+    int rc = VSM_OK;
+    if (b.params.filter) {
+        size_t tot = 0;
+        for (int v = 0; v < n; ++v) {
+            mask_off[v] = tot;
+            tot += (size_t)b.views[v].desc.W * b.views[v].desc.H;
+        }
+        if ((rc = dalloc(c, &d_masks, tot))) return rc;
+        // host tile lists stay alive until the stream is synchronised below
+        std::vector<std::vector<Tile>> filter_tiles(n);
+        for (int v = 0; v < n && !rc; ++v) {
+            // one launch per view so the mask offsets stay simple
+            std::vector<Tile> &vt = filter_tiles[v];
+            tiles_for(b.views[v].desc, 0, vt);
+            Tile *dt = nullptr;
+            if ((rc = dalloc(c, &dt, vt.size()))) break;
+            hipError_t e = hipMemcpyAsync(dt, vt.data(), vt.size() * sizeof(Tile), hipMemcpyHostToDevice,
+                                          c->stream);
+            if (e == hipSuccess)
+                e = launch_filter(b.maxs, b.dviews + v, dt, (int)vt.size(), pc, d_masks + mask_off[v],
+                                  c->stream);
+            dfree(c, dt);                      // back to the cache, stream-ordered
+            if (e != hipSuccess) rc = fail(c, VSM_EDEVICE, "filter launch failed: %s", hipGetErrorString(e));
+        }
+        const hipError_t se = hipStreamSynchronize(c->stream);
+        if (!rc && se != hipSuccess)
+            rc = fail(c, VSM_EDEVICE, "hipStreamSynchronize failed: %s", hipGetErrorString(se));
+    }
+    for (int v = 0; v < n && !rc; ++v)
+        rc = finalize_outputs(c, b.views[v], b.pass, b.ref_ids[v], outs ? &outs[v] : nullptr,
+                              d_masks ? d_masks + mask_off[v] : nullptr);
+    const hipError_t se = hipStreamSynchronize(c->stream);
+    if (!rc && se != hipSuccess)
+        rc = fail(c, VSM_EDEVICE, "hipStreamSynchronize failed: %s", hipGetErrorString(se));
+    dfree(c, d_masks);
+    return rc;
 }
 
 }  // namespace
@@ -819,6 +914,9 @@ int vsm_destroy(vsm_ctx *c)
     if (!c) return VSM_EINVAL;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    for (auto &kv : c->batches)
+        for (auto &g : kv.second->graphs) (void)hipGraphExecDestroy(g.second);
+    c->batches.clear();
     for (auto &kv : c->allocs) (void)hipFree(kv.first);
     c->allocs.clear();
     for (auto &kv : c->cache) (void)hipFree(kv.second);
@@ -860,7 +958,10 @@ int vsm_add_image(vsm_ctx *c, int32_t id, const uint8_t *gray, int32_t w, int32_
     for (int k = 0; k < 16; ++k)
         if (!std::isfinite(P4[k])) return fail(c, VSM_EINVAL, "P4 has non-finite entries");
     HIPCHK(c, hipSetDevice(c->device));
-    vsm_remove_image(c, id);
+    if (c->images.count(id)) {               // replace: the old image must not be in use
+        const int rc = vsm_remove_image(c, id);
+        if (rc) return rc;
+    }
     ImageRec r;
     r.id = id;
     r.W = w;
@@ -884,6 +985,19 @@ int vsm_remove_image(vsm_ctx *c, int32_t id)
     if (!c) return VSM_EINVAL;
     auto it = c->images.find(id);
     if (it == c->images.end()) return VSM_ENOENT;
+    {
+        std::lock_guard<std::mutex> lk(c->serve_mu);
+        auto uses = [id](const ServeReq &q) {
+            return q.ref == id || std::find(q.srcs.begin(), q.srcs.end(), id) != q.srcs.end();
+        };
+        for (const ServeReq *q : c->active)
+            if (uses(*q)) return fail(c, VSM_EINVAL, "image %d is used by an active served request", id);
+        for (const auto &q : c->queue)
+            if (uses(*q)) return fail(c, VSM_EINVAL, "image %d is used by a queued served request", id);
+    }
+    for (const auto &kv : c->batches)
+        for (int rid : kv.second->ref_ids)
+            if (rid == id) return fail(c, VSM_EINVAL, "image %d is a reference of open batch %d", id, kv.first);
     (void)hipStreamSynchronize(c->stream);
     ImageRec &r = it->second;
     dfree(c, r.gray);
@@ -893,7 +1007,6 @@ int vsm_remove_image(vsm_ctx *c, int32_t id)
     dfree(c, r.normal[0]);
     dfree(c, r.normal[1]);
     dfree(c, r.depth4);
-    dfree(c, r.hyp_photo);
     c->images.erase(it);
     return VSM_OK;
 }
@@ -957,31 +1070,57 @@ int vsm_solve(vsm_ctx *c, const vsm_view_req *reqs, int32_t n, int32_t pass, vsm
     if (!reqs || n <= 0) return fail(c, VSM_EINVAL, "no requests");
     if (pass != 0 && pass != 1) return fail(c, VSM_EINVAL, "pass must be 0 or 1");
     HIPCHK(c, hipSetDevice(c->device));
-    std::vector<ViewState> views(n);
-    std::vector<const vsm_view_req *> rv(n);
-    int rc = VSM_OK;
-    int prepared = 0;
-    for (int v = 0; v < n && !rc; ++v) {
-        rv[v] = &reqs[v];
-        rc = prepare_view(c, reqs[v], pass, (uint32_t)reqs[v].ref_id, views[v]);
-        if (!rc) ++prepared;
-    }
-    uint32_t *d_masks = nullptr;
-    std::vector<size_t> mask_off(n, 0);
-    if (!rc && c->params.filter) {
-        size_t tot = 0;
-        for (int v = 0; v < n; ++v) { mask_off[v] = tot; tot += (size_t)views[v].desc.W * views[v].desc.H; }
-        rc = dalloc(c, &d_masks, tot);
-    }
-    if (!rc) rc = run_views(c, views, rv, pass, d_masks, mask_off, use_graph);
-    for (int v = 0; v < n && !rc; ++v)
-        rc = finalize_outputs(c, views[v], pass, reqs[v].ref_id, outs ? &outs[v] : nullptr,
-                              d_masks ? d_masks + mask_off[v] : nullptr);
-    const hipError_t se = hipStreamSynchronize(c->stream);
-    if (!rc && se != hipSuccess)
-        rc = fail(c, VSM_EDEVICE, "hipStreamSynchronize failed: %s", hipGetErrorString(se));
-    for (int v = 0; v < prepared; ++v) release_view(c, views[v]);
-    dfree(c, d_masks);
+    Batch b;
+    int rc = batch_begin(c, reqs, n, pass, use_graph, b);
+    if (rc) return rc;
+    rc = batch_step(c, b, b.nsteps);
+    if (!rc) rc = batch_finish(c, b, outs);
+    batch_release(c, b);
+    return rc;
+}
+
+int vsm_batch_begin(vsm_ctx *c, const vsm_view_req *reqs, int32_t n, int32_t pass, int32_t use_graph,
+                    int32_t *batch)
+{
+    if (!c || !batch) return VSM_EINVAL;
+    *batch = 0;
+    if (!reqs || n <= 0) return fail(c, VSM_EINVAL, "no requests");
+    if (pass != 0 && pass != 1) return fail(c, VSM_EINVAL, "pass must be 0 or 1");
+    HIPCHK(c, hipSetDevice(c->device));
+    auto b = std::make_unique<Batch>();
+    const int rc = batch_begin(c, reqs, n, pass, use_graph, *b);
+    if (rc) return rc;
+    const int id = c->next_batch++;
+    c->batches[id] = std::move(b);
+    *batch = id;
+    return VSM_OK;
+}
+
+int vsm_batch_step(vsm_ctx *c, int32_t batch, int32_t n_halfsteps, int32_t *remaining)
+{
+    if (!c) return VSM_EINVAL;
+    auto it = c->batches.find(batch);
+    if (it == c->batches.end()) return fail(c, VSM_ENOENT, "unknown batch %d", batch);
+    if (n_halfsteps < 0) return fail(c, VSM_EINVAL, "negative half-step count");
+    HIPCHK(c, hipSetDevice(c->device));
+    Batch &b = *it->second;
+    const int rc = batch_step(c, b, n_halfsteps);
+    if (remaining) *remaining = b.nsteps - b.steps_done;
+    return rc;
+}
+
+int vsm_batch_end(vsm_ctx *c, int32_t batch, vsm_view_out *outs, int32_t discard)
+{
+    if (!c) return VSM_EINVAL;
+    auto it = c->batches.find(batch);
+    if (it == c->batches.end()) return fail(c, VSM_ENOENT, "unknown batch %d", batch);
+    Batch &b = *it->second;
+    if (!discard && b.steps_done < b.nsteps)
+        return fail(c, VSM_EINVAL, "batch %d has %d half-steps left", batch, b.nsteps - b.steps_done);
+    HIPCHK(c, hipSetDevice(c->device));
+    const int rc = discard ? VSM_OK : batch_finish(c, b, outs);
+    batch_release(c, b);
+    c->batches.erase(it);
     return rc;
 }
 
@@ -1067,7 +1206,7 @@ static int debug_state(vsm_ctx *c, const vsm_view_req *req, int pass, int mode, 
             depth[i] = h[i].x;
             normal[3 * i] = h[i].y; normal[3 * i + 1] = h[i].z; normal[3 * i + 2] = h[i].w;
         }
-        if (mode == 1) rc = read_counters(c);
+        if (mode == 1) rc = read_counters(c, pc);
     }
     dfree(c, dv); dfree(c, dt); dfree(c, dm);
     release_view(c, vs);
@@ -1421,21 +1560,239 @@ int vsm_median_blur3(vsm_ctx *c, const float *in, int32_t e_size, int32_t n_size
 }
 
 // ---------------------------------------------------------------------------
-// Serving: continuous batching of photometric solves.
+// Serving: continuous batching of per-view solves, photometric and (once a
+// view's photometric result and its sources' photometric depth maps are
+// resident) geometric consistency.
+//
+// Results are those of running the requests one at a time in ticket order: a
+// geometric request waits for every earlier photometric request of its
+// reference or of one of its sources, a photometric request for every earlier
+// request that reads or writes its image's photometric maps.  Within those
+// constraints requests share launches, and a request's result does not depend
+// on its co-runners (tiles are independent within a half-step and the random
+// streams are keyed by view and pixel).
 // ---------------------------------------------------------------------------
-// Device bytes a request needs while active: state, outputs, and the image's
-// photometric depth map that keeps its result if the image has none yet.
-static size_t serve_footprint(vsm_ctx *c, const ServeReq &r)
+
+// Scratch the loop allocates for a request besides its state: its share of
+// the served tile list (reallocated at twice the size needed), the filter's
+// and the init's tile lists and descriptors, and block rounding.
+static size_t serve_scratch_bytes(int W, int H)
 {
-    const ImageRec &ref = c->images[r.ref];
-    const size_t hw = (size_t)ref.W * ref.H;
-    return state_bytes(ref.W, ref.H, (int)r.srcs.size()) + hw * 20 + (ref.depth[0] ? 0 : hw * 4) +
-           4096;
+    const size_t ntiles = (size_t)((W + kTileW - 1) / kTileW) * ((H + kTileH - 1) / kTileH);
+    return 4 * ntiles * sizeof(Tile) + 2 * sizeof(ViewDesc) + 16 * kAllocAlign;
 }
 
-int vsm_submit(vsm_ctx *c, const vsm_view_req *req, uint64_t *ticket)
+// Device bytes a request needs while active: state, outputs (depth, normal,
+// cost, consistency mask), the image maps its result is kept in when the image
+// has none yet, the packed depth textures of its sources (geometric pass) that
+// are not resident, and its scratch.  worst: as if nothing were resident.
+static size_t serve_footprint(vsm_ctx *c, const ServeReq &r, bool worst)
+{
+    const ImageRec &ref = c->images.at(r.ref);
+    const size_t hw = (size_t)ref.W * ref.H;
+    size_t b = state_bytes(ref.W, ref.H, (int)r.srcs.size()) + hw * 24 + serve_scratch_bytes(ref.W, ref.H);
+    if (worst || !ref.depth[r.pass]) b += hw * 4;
+    if (worst || !ref.normal[r.pass]) b += hw * 12;
+    if (r.pass == 1)
+        for (int s : r.srcs) {
+            const ImageRec &S = c->images.at(s);
+            if (worst || !S.depth4) b += (size_t)S.W * S.H * sizeof(float4);
+        }
+    return b;
+}
+
+static bool has_id(const std::vector<int> &v, int id)
+{
+    return std::find(v.begin(), v.end(), id) != v.end();
+}
+
+// Whether the pending request e (earlier or later) must finish before r starts.
+static bool serve_blocks(const ServeReq &e, const ServeReq &r)
+{
+    if (e.ticket >= r.ticket) return false;
+    if (r.pass == 0)          // r rewrites its image's photometric maps
+        return e.ref == r.ref || (e.pass == 1 && has_id(e.srcs, r.ref));
+    if (e.pass == 0)          // r reads e's result
+        return e.ref == r.ref || has_id(r.srcs, e.ref);
+    return e.ref == r.ref;    // both write the geometric maps
+}
+
+// 1: r can start now; 0: it waits for an earlier request; negative: it can
+// never start (code and message set).
+static int serve_ready(vsm_ctx *c, const ServeReq &r)
+{
+    for (const ServeReq *e : c->active)
+        if (serve_blocks(*e, r)) return 0;
+    for (const auto &e : c->queue)
+        if (serve_blocks(*e, r)) return 0;
+    auto ri = c->images.find(r.ref);
+    if (ri == c->images.end()) return fail(c, VSM_ENOENT, "unknown reference image %d", r.ref);
+    for (int s : r.srcs)
+        if (c->images.find(s) == c->images.end()) return fail(c, VSM_ENOENT, "unknown source image %d", s);
+    if (r.pass == 1) {
+        if (!ri->second.depth[0] || !ri->second.normal[0])
+            return fail(c, VSM_EINVAL, "geometric request for image %d: it has no photometric depth and "
+                        "normal maps", r.ref);
+        for (int s : r.srcs)
+            if (!c->images.at(s).depth[0])
+                return fail(c, VSM_EINVAL, "geometric request for image %d: source %d has no photometric "
+                            "depth map", r.ref, s);
+    }
+    return 1;
+}
+
+static void serve_drop_keep(vsm_ctx *c, ServeReq &r)
+{
+    dfree(c, r.keep_depth);
+    dfree(c, r.keep_normal);
+    r.keep_depth = nullptr;
+    r.keep_normal = nullptr;
+}
+
+// State, outputs and initial hypotheses of an admitted request; on failure
+// everything it allocated is released again.
+static int serve_start(vsm_ctx *c, ServeReq &r)
+{
+    vsm_view_req q;
+    memset(&q, 0, sizeof q);
+    q.ref_id = r.ref;
+    q.n_src = (int)r.srcs.size();
+    q.src_ids = r.srcs.data();
+    q.depth_min = r.dmin;
+    q.depth_max = r.dmax;
+    int rc = prepare_view(c, q, r.pass, (uint32_t)r.ref, r.vs);
+    if (rc) return rc;
+    const ImageRec &img = c->images.at(r.ref);
+    const size_t hw = (size_t)r.vs.desc.W * r.vs.desc.H;
+    std::vector<Tile> tl;
+    tiles_for(r.vs.desc, 0, tl);
+    ViewDesc *dv = nullptr;
+    Tile *dt = nullptr;
+    rc = dalloc(c, &r.out_depth, hw * 6);
+    if (!rc && !img.depth[r.pass]) rc = dalloc(c, &r.keep_depth, hw);
+    if (!rc && !img.normal[r.pass]) rc = dalloc(c, &r.keep_normal, hw * 3);
+    if (!rc) rc = dalloc(c, &dv, 1);
+    if (!rc) rc = dalloc(c, &dt, tl.size());
+    if (!rc) {
+        const PMConst pc = make_const(c->params);
+        hipError_t e = hipMemcpyAsync(dv, &r.vs.desc, sizeof(ViewDesc), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(dt, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess && r.pass == 1)     // start from the photometric depth and normals
+            e = launch_load_hyp(dv, 0, (int)hw, img.depth[0], img.normal[0], pc.cos_max_normal, c->stream);
+        if (e == hipSuccess) e = launch_init(dv, dt, (int)tl.size(), pc, r.pass == 0 ? 1 : 0, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = fail(c, VSM_EDEVICE, "request start failed: %s", hipGetErrorString(e));
+    }
+    dfree(c, dv);
+    dfree(c, dt);
+    if (rc) {
+        dfree(c, r.out_depth);
+        r.out_depth = nullptr;
+        serve_drop_keep(c, r);
+        release_view(c, r.vs);
+        return rc;
+    }
+    r.out_normal = r.out_depth + hw;
+    r.out_cost = r.out_depth + 4 * hw;
+    r.out_mask = reinterpret_cast<uint32_t *>(r.out_depth + 5 * hw);
+    r.host_step = 0;
+    return VSM_OK;
+}
+
+static void serve_fail_req(vsm_ctx *c, std::unique_ptr<ServeReq> owned, int code)
+{
+    owned->done = true;
+    owned->error = code;
+    owned->errmsg = c->err;
+    c->reqs[owned->ticket] = std::move(owned);
+}
+
+// Admit, in ticket order, every request that can start and fits the budget.
+// A request that waits for an earlier one is passed over; one that is ready
+// but does not fit stops admission (first come, first served for memory),
+// unless nothing is active: then nothing will free memory and it fails.
+static int serve_admit(vsm_ctx *c)
+{
+    auto it = c->queue.begin();
+    while (it != c->queue.end() && (int)c->active.size() < kServeMaxViews) {
+        ServeReq &r = **it;
+        const int ready = serve_ready(c, r);
+        if (ready < 0) {
+            serve_fail_req(c, std::move(*it), ready);
+            it = c->queue.erase(it);
+            continue;
+        }
+        if (ready == 0) {
+            ++it;
+            continue;
+        }
+        const size_t fp = serve_footprint(c, r, false);
+        if (c->budget && c->in_use + fp > c->budget) {
+            if (!c->active.empty()) break;
+            fail(c, VSM_ENOMEM, "request for image %d needs %zu bytes; %llu of the %llu-byte budget "
+                 "are held by resident images and maps", r.ref, fp, (unsigned long long)c->in_use,
+                 (unsigned long long)c->budget);
+            serve_fail_req(c, std::move(*it), VSM_ENOMEM);
+            it = c->queue.erase(it);
+            continue;
+        }
+        const int rc = serve_start(c, r);
+        if (rc == VSM_ENOMEM && !c->active.empty()) break;     // memory frees up later
+        if (rc) {
+            if (rc == VSM_EDEVICE) return rc;
+            serve_fail_req(c, std::move(*it), rc);
+            it = c->queue.erase(it);
+            continue;
+        }
+        c->active.push_back(&r);
+        c->reqs[r.ticket] = std::move(*it);
+        it = c->queue.erase(it);
+    }
+    return VSM_OK;
+}
+
+// Filter (when the request asks for it) and outputs of a request whose last
+// half-step has run; dv is its descriptor slot in the served view array.
+static int serve_finish(vsm_ctx *c, ServeReq &r, ViewDesc *dv)
+{
+    ImageRec &img = c->images.at(r.ref);
+    const size_t hw = (size_t)r.vs.desc.W * r.vs.desc.H;
+    const PMConst pc = make_const(c->params);
+    r.vs.desc.step = r.vs.desc.num_steps;
+    HIPCHK(c, hipMemcpyAsync(dv, &r.vs.desc, sizeof(ViewDesc), hipMemcpyHostToDevice, c->stream));
+    if (r.filter) {
+        std::vector<Tile> tl;
+        tiles_for(r.vs.desc, 0, tl);
+        Tile *dt = nullptr;
+        const int rc = dalloc(c, &dt, tl.size());
+        if (rc) return rc;
+        hipError_t e = hipMemcpyAsync(dt, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            e = launch_filter(r.vs.desc.n_src, dv, dt, (int)tl.size(), pc, r.out_mask, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        dfree(c, dt);
+        if (e != hipSuccess) return fail(c, VSM_EDEVICE, "filter failed: %s", hipGetErrorString(e));
+    } else {
+        HIPCHK(c, launch_fill_u32(r.out_mask, (long long)hw, 0u, c->stream));
+    }
+    // the result also becomes the image's depth and normal map of this pass
+    // on the device (fusion, later geometric passes), as after vsm_solve
+    if (!img.depth[r.pass]) { img.depth[r.pass] = r.keep_depth; r.keep_depth = nullptr; }
+    if (!img.normal[r.pass]) { img.normal[r.pass] = r.keep_normal; r.keep_normal = nullptr; }
+    HIPCHK(c, launch_finalize(dv, 0, (int)hw, r.out_depth, r.out_normal, r.out_cost, img.depth[r.pass],
+                              c->stream));
+    HIPCHK(c, hipMemcpyAsync(img.normal[r.pass], r.out_normal, hw * 12, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (r.pass == 0) ++img.depth_ver;
+    return VSM_OK;
+}
+
+static int serve_submit(vsm_ctx *c, const vsm_view_req *req, int pass, int filter, uint64_t *ticket)
 {
     if (!c || !req || !ticket) return VSM_EINVAL;
+    if (pass != 0 && pass != 1) return fail(c, VSM_EINVAL, "pass must be 0 or 1");
+    if (filter < -1 || filter > 1) return fail(c, VSM_EINVAL, "filter must be -1, 0 or 1");
     if (req->n_src < 1 || req->n_src > VSM_MAX_SRC || !req->src_ids)
         return fail(c, VSM_EINVAL, "n_src %d outside [1, %d]", req->n_src, VSM_MAX_SRC);
     if (c->images.find(req->ref_id) == c->images.end())
@@ -1444,77 +1801,35 @@ int vsm_submit(vsm_ctx *c, const vsm_view_req *req, uint64_t *ticket)
         if (c->images.find(req->src_ids[s]) == c->images.end())
             return fail(c, VSM_ENOENT, "unknown source image %d", req->src_ids[s]);
     std::lock_guard<std::mutex> lk(c->serve_mu);
+    HIPCHK(c, hipSetDevice(c->device));
+    // resident serving scratch is allocated before any admission decision
+    int rc = VSM_OK;
+    if (!c->serve_views && (rc = dalloc(c, &c->serve_views, kServeMaxViews))) return rc;
+    if ((rc = ensure_c0(c, req->n_src))) return rc;
     auto r = std::make_unique<ServeReq>();
-    r->ticket = c->next_ticket++;
     r->ref = req->ref_id;
+    r->pass = pass;
+    r->filter = filter < 0 ? (c->params.filter ? 1 : 0) : filter;
     r->srcs.assign(req->src_ids, req->src_ids + req->n_src);
     r->dmin = req->depth_min;
     r->dmax = req->depth_max;
-    r->footprint = serve_footprint(c, *r);
-    if (c->budget && r->footprint > c->budget)
-        return fail(c, VSM_ENOMEM, "request footprint %zu exceeds the HBM budget", r->footprint);
+    const size_t worst = serve_footprint(c, *r, true);
+    if (c->budget && worst > c->budget)
+        return fail(c, VSM_ENOMEM, "request footprint %zu exceeds the HBM budget", worst);
+    r->ticket = c->next_ticket++;
     *ticket = r->ticket;
     c->queue.push_back(std::move(r));
     return VSM_OK;
 }
 
-static int serve_admit(vsm_ctx *c)
+int vsm_submit(vsm_ctx *c, const vsm_view_req *req, uint64_t *ticket)
 {
-    while (!c->queue.empty() && (int)c->active.size() < kServeMaxViews) {
-        ServeReq &r = *c->queue.front();
-        if (c->budget && c->in_use + r.footprint > c->budget) break;   // wait for memory
-        vsm_view_req q;
-        memset(&q, 0, sizeof q);
-        q.ref_id = r.ref;
-        q.n_src = (int)r.srcs.size();
-        q.src_ids = r.srcs.data();
-        q.depth_min = r.dmin;
-        q.depth_max = r.dmax;
-        int rc = prepare_view(c, q, 0, (uint32_t)r.ref, r.vs);
-        std::unique_ptr<ServeReq> owned = std::move(c->queue.front());
-        c->queue.pop_front();
-        if (rc == VSM_ENOMEM) {           // budget raced: requeue at the front
-            c->queue.push_front(std::move(owned));
-            break;
-        }
-        if (rc) {
-            owned->done = true;
-            owned->error = rc;
-            owned->errmsg = c->err;
-            c->reqs[owned->ticket] = std::move(owned);
-            continue;
-        }
-        const size_t hw = (size_t)r.vs.desc.W * r.vs.desc.H;
-        rc = dalloc(c, &r.out_depth, hw * 5);
-        ImageRec &img = c->images[r.ref];
-        if (!rc && !img.depth[0]) {
-            rc = dalloc(c, &img.depth[0], hw);     // where the result is kept
-            if (rc) { dfree(c, r.out_depth); r.out_depth = nullptr; }
-        }
-        if (rc) {
-            release_view(c, r.vs);
-            c->queue.push_front(std::move(owned));
-            break;
-        }
-        r.out_normal = r.out_depth + hw;
-        r.out_cost = r.out_depth + 4 * hw;
-        r.host_step = 0;
-        c->active.push_back(owned.get());
-        c->reqs[owned->ticket] = std::move(owned);
-        // random initial hypotheses for the new view
-        ViewDesc *dv = nullptr;
-        Tile *dt = nullptr;
-        std::vector<Tile> tl;
-        tiles_for(r.vs.desc, 0, tl);
-        if ((rc = dalloc(c, &dv, 1)) || (rc = dalloc(c, &dt, tl.size()))) { dfree(c, dv); return rc; }
-        HIPCHK(c, hipMemcpyAsync(dv, &r.vs.desc, sizeof(ViewDesc), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(dt, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, launch_init(dv, dt, (int)tl.size(), make_const(c->params), 1, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        dfree(c, dv);
-        dfree(c, dt);
-    }
-    return VSM_OK;
+    return serve_submit(c, req, 0, -1, ticket);
+}
+
+int vsm_submit_pass(vsm_ctx *c, const vsm_view_req *req, int32_t pass, int32_t filter, uint64_t *ticket)
+{
+    return serve_submit(c, req, pass, filter, ticket);
 }
 
 int vsm_serve_step(vsm_ctx *c, int32_t *n_active, int32_t *n_queued)
@@ -1526,9 +1841,6 @@ int vsm_serve_step(vsm_ctx *c, int32_t *n_active, int32_t *n_queued)
     if (rc) return rc;
     const int na = (int)c->active.size();
     if (na > 0) {
-        if (!c->serve_views) {
-            if ((rc = dalloc(c, &c->serve_views, kServeMaxViews))) return rc;
-        }
         std::vector<ViewDesc> hv(na);
         std::vector<Tile> tiles;
         int maxs = 1;
@@ -1546,50 +1858,42 @@ int vsm_serve_step(vsm_ctx *c, int32_t *n_active, int32_t *n_queued)
             if ((rc = dalloc(c, &c->serve_tiles, tiles.size() * 2))) return rc;
             c->serve_cap_tiles = (int)tiles.size() * 2;
         }
+        const PMConst pc = make_const(c->params);
         HIPCHK(c, hipMemcpyAsync(c->serve_views, hv.data(), na * sizeof(ViewDesc), hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->serve_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream));
-        if ((rc = record_halfstep(c, maxs, c->serve_views, c->serve_tiles, (int)tiles.size(),
-                                  make_const(c->params), c->timing)))
+        if ((rc = record_halfstep(c, maxs, c->serve_views, c->serve_tiles, (int)tiles.size(), pc, c->timing)))
             return rc;
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        // retire finished views
+        // Every active request has advanced one half-step.  The finished ones
+        // leave the active list before any fallible work on them, so a
+        // failure below affects only the request it belongs to.
+        std::vector<std::pair<ServeReq *, int>> retire;
         std::vector<ServeReq *> still;
         for (int v = 0; v < na; ++v) {
             ServeReq &r = *c->active[v];
             r.host_step += 1;
-            if (r.host_step < r.vs.desc.num_steps) { still.push_back(&r); continue; }
-            r.vs.desc.step = r.host_step;
-            const size_t hw = (size_t)r.vs.desc.W * r.vs.desc.H;
-            const PMConst pc = make_const(c->params);
-            ViewDesc *dv = c->serve_views + v;
-            HIPCHK(c, hipMemcpyAsync(dv, &r.vs.desc, sizeof(ViewDesc), hipMemcpyHostToDevice, c->stream));
-            if (c->params.filter) {
-                std::vector<Tile> tl;
-                tiles_for(r.vs.desc, 0, tl);
-                Tile *dt = nullptr;
-                if ((rc = dalloc(c, &dt, tl.size()))) return rc;
-                HIPCHK(c, hipMemcpyAsync(dt, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream));
-                HIPCHK(c, launch_filter(r.vs.desc.n_src, dv, dt, (int)tl.size(), pc, nullptr, c->stream));
-                HIPCHK(c, hipStreamSynchronize(c->stream));
-                dfree(c, dt);
-            }
-            // the result also becomes the image's photometric depth map on the
-            // device (for fusion and later geometric passes), as with vsm_solve
-            auto im = c->images.find(r.ref);
-            float *keep = nullptr;
-            if (im != c->images.end() && im->second.depth[0]) {   // allocated at admission
-                keep = im->second.depth[0];
-                ++im->second.depth_ver;
-            }
-            HIPCHK(c, launch_finalize(dv, 0, (int)hw, r.out_depth, r.out_normal, r.out_cost, keep,
-                                      c->stream));
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-            release_view(c, r.vs);
-            r.done = true;
+            if (r.host_step < r.vs.desc.num_steps) still.push_back(&r);
+            else retire.emplace_back(&r, v);
         }
         c->active.swap(still);
-        rc = read_counters(c);
+        int dev_err = VSM_OK;
+        for (auto &rv : retire) {
+            ServeReq &r = *rv.first;
+            const int e = serve_finish(c, r, c->serve_views + rv.second);
+            release_view(c, r.vs);
+            serve_drop_keep(c, r);
+            r.done = true;
+            if (e) {
+                r.error = e;
+                r.errmsg = c->err;
+                dfree(c, r.out_depth);
+                r.out_depth = nullptr;
+                if (e == VSM_EDEVICE && !dev_err) dev_err = e;
+            }
+        }
+        rc = read_counters(c, pc);
         if (rc) return rc;
+        if (dev_err) return dev_err;
     }
     if (n_active) *n_active = (int)c->active.size();
     if (n_queued) *n_queued = (int)c->queue.size();
@@ -1615,15 +1919,24 @@ int vsm_poll(vsm_ctx *c, uint64_t ticket, vsm_view_out *out, int32_t *done)
         c->reqs.erase(it);
         return e;
     }
-    const ImageRec &ref = c->images[r.ref];
-    const size_t hw = (size_t)ref.W * ref.H;
+    const size_t hw = (size_t)r.vs.desc.W * r.vs.desc.H;
     if (out && out->depth) HIPCHK(c, hipMemcpyAsync(out->depth, r.out_depth, hw * 4, hipMemcpyDeviceToHost, c->stream));
     if (out && out->normal) HIPCHK(c, hipMemcpyAsync(out->normal, r.out_normal, hw * 12, hipMemcpyDeviceToHost, c->stream));
     if (out && out->cost) HIPCHK(c, hipMemcpyAsync(out->cost, r.out_cost, hw * 4, hipMemcpyDeviceToHost, c->stream));
-    if (out && out->consistency) memset(out->consistency, 0, hw * 4);
+    if (out && out->consistency)
+        HIPCHK(c, hipMemcpyAsync(out->consistency, r.out_mask, hw * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     dfree(c, r.out_depth);
     c->reqs.erase(it);
+    return VSM_OK;
+}
+
+int vsm_set_hbm_budget(vsm_ctx *c, uint64_t hbm_budget_bytes)
+{
+    if (!c) return VSM_EINVAL;
+    std::lock_guard<std::mutex> lk(c->serve_mu);
+    c->budget = hbm_budget_bytes;
+    if (c->budget) release_cached(c, 0);
     return VSM_OK;
 }
 

@@ -59,6 +59,23 @@ struct ViewDesc {
     double Mb[kMaxSrc][12];         // src (x,y,1,d) -> ref homogeneous pixel
 };
 
+// Integer anchor of one source for one tile: the texel nearest the projection
+// (qx, qy, qz) of the tile centre at mid-depth, clamped into the W x H image
+// (a projection behind the camera, at infinity or NaN gives a point inside
+// it).  The window taps are formed relative to the anchor and, wherever a
+// window may leave the image, clamped to the image box, so every relative
+// texel coordinate a gather uses lies within the image side (< 2^15): the
+// signed 24-bit products of the gather offsets cannot wrap.
+__host__ __device__ inline void source_anchor(double qx, double qy, double qz, int W, int H, int &ax,
+                                              int &ay)
+{
+    double px = qz > 0.0 ? qx / qz : 0.0, py = qz > 0.0 ? qy / qz : 0.0;
+    px = fmin(fmax(px, 0.0), (double)(W - 1));      // fmax(NaN, 0) = 0
+    py = fmin(fmax(py, 0.0), (double)(H - 1));
+    ax = (int)rint(px);
+    ay = (int)rint(py);
+}
+
 // Work item: one tile of one view.
 struct Tile {
     int view, x0, y0, pad;

@@ -60,10 +60,8 @@ __device__ void setup_source(const ViewDesc &V, int s, int uc, int vc, SrcLds &o
     const double qx = M[0] * uc + M[1] * vc + M[2] + M[3] * dmid;
     const double qy = M[4] * uc + M[5] * vc + M[6] + M[7] * dmid;
     const double qz = M[8] * uc + M[9] * vc + M[10] + M[11] * dmid;
-    double px = qz > 0.0 ? qx / qz : 0.0, py = qz > 0.0 ? qy / qz : 0.0;
-    px = fmin(fmax(px, -1.0e7), 1.0e7);
-    py = fmin(fmax(py, -1.0e7), 1.0e7);
-    const int ax = (int)rint(px), ay = (int)rint(py);
+    int ax, ay;
+    source_anchor(qx, qy, qz, V.src_w[s], V.src_h[s], ax, ay);
     double A[12], B[12];
     for (int r = 0; r < 3; ++r) {
         A[4 * r + 0] = M[4 * r + 0];

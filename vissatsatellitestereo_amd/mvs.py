@@ -138,14 +138,10 @@ class MvsContext:
         self._check(self.lib.vsm_set_depth_map(self.h, int(image_id), int(kind), fptr(d), w, h))
 
     # ------------------------------------------------------------------
-    def solve(self, views, pass_=0, want=("depth", "normal", "cost", "consistency"),
-              use_graph=False):
-        """views: list of dicts with ref, srcs, depth_range and optionally
-        init_depth / init_normal.  Returns one dict of outputs per view."""
+    def _reqs(self, views):
         n = len(views)
         reqs = (_lib.ViewReq * n)()
-        outs = (_lib.ViewOut * n)()
-        keep, results = [], []
+        keep = []
         for i, v in enumerate(views):
             r, k = make_req(v["ref"], v["srcs"], v["depth_range"], v.get("init_depth"),
                             v.get("init_normal"))
@@ -153,6 +149,12 @@ class MvsContext:
             keep.append(k)
             if int(v["ref"]) not in self.sizes:
                 raise VsmError(-2, f"unknown reference image {int(v['ref'])}")
+        return reqs, keep
+
+    def _outputs(self, views, want):
+        outs = (_lib.ViewOut * len(views))()
+        results = []
+        for i, v in enumerate(views):
             h, w = self.sizes[int(v["ref"])]
             res = {}
             if "depth" in want:
@@ -168,8 +170,26 @@ class MvsContext:
                 res["consistency"] = np.zeros((h, w), np.uint32)
                 outs[i].consistency = u32ptr(res["consistency"])
             results.append(res)
-        self._check(self.lib.vsm_solve(self.h, reqs, n, int(pass_), outs, 1 if use_graph else 0))
+        return outs, results
+
+    def solve(self, views, pass_=0, want=("depth", "normal", "cost", "consistency"),
+              use_graph=False):
+        """views: list of dicts with ref, srcs, depth_range and optionally
+        init_depth / init_normal.  Returns one dict of outputs per view."""
+        reqs, keep = self._reqs(views)
+        outs, results = self._outputs(views, want)
+        self._check(self.lib.vsm_solve(self.h, reqs, len(views), int(pass_), outs,
+                                       1 if use_graph else 0))
         return results
+
+    def batch(self, views, pass_=0, use_graph=False):
+        """A staged solve (vsm_batch_begin): step() runs half-steps, end()
+        returns the outputs as solve() does."""
+        reqs, keep = self._reqs(views)
+        b = ctypes.c_int32()
+        self._check(self.lib.vsm_batch_begin(self.h, reqs, len(views), int(pass_),
+                                             1 if use_graph else 0, ctypes.byref(b)))
+        return SolveBatch(self, b.value, views)
 
     def eval_cost(self, ref, srcs, depth_range, depth, normal_uvd, pass_=0):
         r, keep = make_req(ref, srcs, depth_range)
@@ -263,11 +283,17 @@ class MvsContext:
         return out
 
     # ------------------------------------------------------------------
-    def submit(self, ref, srcs, depth_range):
+    def submit(self, ref, srcs, depth_range, pass_=0, filter=None):
+        """Queue a served solve (vsm_submit_pass); filter None follows the
+        context's filter parameter."""
         r, keep = make_req(ref, srcs, depth_range)
         t = ctypes.c_uint64()
-        self._check(self.lib.vsm_submit(self.h, ctypes.byref(r), ctypes.byref(t)))
+        f = -1 if filter is None else (1 if filter else 0)
+        self._check(self.lib.vsm_submit_pass(self.h, ctypes.byref(r), int(pass_), f, ctypes.byref(t)))
         return t.value
+
+    def set_hbm_budget(self, nbytes):
+        self._check(self.lib.vsm_set_hbm_budget(self.h, int(nbytes)))
 
     def serve_step(self):
         na, nq = ctypes.c_int32(), ctypes.c_int32()
@@ -287,6 +313,9 @@ class MvsContext:
         if "cost" in want:
             res["cost"] = np.zeros((h, w), np.float32)
             out.cost = fptr(res["cost"])
+        if "consistency" in want:
+            res["consistency"] = np.zeros((h, w), np.uint32)
+            out.consistency = u32ptr(res["consistency"])
         done = ctypes.c_int32()
         self._check(self.lib.vsm_poll(self.h, ctypes.c_uint64(ticket), ctypes.byref(out),
                                       ctypes.byref(done)))
@@ -306,3 +335,30 @@ class MvsContext:
 
     def synchronize(self):
         self._check(self.lib.vsm_synchronize(self.h))
+
+
+class SolveBatch:
+    """Handle of a staged solve (vsm_batch_begin / _step / _end)."""
+
+    def __init__(self, ctx, handle, views):
+        self.ctx, self.handle, self.views = ctx, handle, list(views)
+        self.remaining = None
+
+    def step(self, n_halfsteps=1):
+        """Run up to n more half-steps over every view; returns how many are left."""
+        rem = ctypes.c_int32()
+        self.ctx._check(self.ctx.lib.vsm_batch_step(self.ctx.h, self.handle, int(n_halfsteps),
+                                                    ctypes.byref(rem)))
+        self.remaining = rem.value
+        return rem.value
+
+    def end(self, want=("depth", "normal", "cost", "consistency")):
+        outs, results = self.ctx._outputs(self.views, want)
+        h, self.handle = self.handle, None
+        self.ctx._check(self.ctx.lib.vsm_batch_end(self.ctx.h, h, outs, 0))
+        return results
+
+    def discard(self):
+        if self.handle is not None:
+            h, self.handle = self.handle, None
+            self.ctx._check(self.ctx.lib.vsm_batch_end(self.ctx.h, h, None, 1))

@@ -4,7 +4,7 @@ workspace plumbing they have in common.
 The reference runs `colmap patch_match_stereo` as a subprocess per call
 (lib/run_cmd.py:44), so nothing survives between the photometric call, the
 consistency call and fusion except files.  Here the three share one device
-context: images, photometric hypotheses and depth maps stay resident in HBM
+context: images and depth/normal maps stay resident in HBM
 between calls of the same process, and everything is still written to the
 workspace files, so a fresh process resumes from them exactly as the
 reference does.

@@ -11,6 +11,13 @@ each at its own iteration.  Requests that arrive while others run join as
 slots free up; a request's result does not depend on its co-runners (per-view
 counter-based random streams, tiles independent within a half-step).
 
+With ``geometric=True`` a request is a view's full solve: its photometric
+pass is submitted on arrival (no filter, as the reference's photometric call,
+colmap_mvs_commands.py:39-55), and its geometric-consistency pass with the
+filter (colmap_mvs_commands.py:58-82) as soon as the photometric passes of the
+view and of all its sources have been submitted; the engine starts it once
+they have finished.  The request completes with the geometric result.
+
 ``run_schedule`` replays an arrival schedule against wall-clock time and
 reports p50/p99 latency (arrival -> result available) and views/s.
 """
@@ -32,26 +39,58 @@ def percentile(values, q):
 
 
 class ServeLoop:
-    def __init__(self, engine, clock=time.perf_counter, want=("depth",), on_complete=None):
+    def __init__(self, engine, clock=time.perf_counter, want=("depth",), on_complete=None,
+                 geometric=False):
         self.engine = engine
         self.clock = clock
         self.want = want
         self.on_complete = on_complete   # f(ticket, view, result) when a request finishes
-        self.pending = {}      # ticket -> (view, arrival time)
-        self.done = {}         # ticket -> (result, arrival, completion)
+        self.geometric = geometric
+        self.pending = {}      # request ticket -> (view, arrival time)
+        self.done = {}         # request ticket -> (result, arrival, completion)
+        self.photo_submitted = set()     # image ids whose photometric pass was submitted
+        self.awaiting_geo = {}           # request ticket -> view (geometric pass not yet submitted)
+        self.engine_ticket = {}          # request ticket -> engine ticket it completes with
+        self.inner = {}                  # engine ticket of a photometric pass -> ref (two-pass mode)
 
     def submit(self, view, arrival=None):
-        t = self.engine.submit(view["ref"], view["srcs"], view["depth_range"])
-        self.pending[t] = (view, self.clock() if arrival is None else arrival)
+        """Submit a request; returns its ticket (the engine ticket of its
+        first pass)."""
+        arr = self.clock() if arrival is None else arrival
+        if not self.geometric:
+            t = self.engine.submit(view["ref"], view["srcs"], view["depth_range"])
+            self.pending[t] = (view, arr)
+            self.engine_ticket[t] = t
+            return t
+        t = self.engine.submit(view["ref"], view["srcs"], view["depth_range"], pass_=0, filter=False)
+        self.inner[t] = view["ref"]
+        self.photo_submitted.add(int(view["ref"]))
+        self.pending[t] = (view, arr)
+        self.awaiting_geo[t] = view
+        self._release_geometric()
         return t
+
+    def _release_geometric(self):
+        for t, view in list(self.awaiting_geo.items()):
+            if all(int(s) in self.photo_submitted for s in view["srcs"]):
+                g = self.engine.submit(view["ref"], view["srcs"], view["depth_range"], pass_=1,
+                                       filter=True)
+                self.engine_ticket[t] = g
+                del self.awaiting_geo[t]
 
     def step(self):
         """One batched half-step; collects finished requests."""
         n_active, n_queued = self.engine.serve_step()
         now = self.clock()
+        for e, ref in list(self.inner.items()):          # photometric passes: release
+            if self.engine.poll(e, ref, ()) is not None:
+                del self.inner[e]
         for t in list(self.pending):
+            e = self.engine_ticket.get(t)
+            if e is None or e in self.inner:
+                continue
             view, arr = self.pending[t]
-            res = self.engine.poll(t, view["ref"], self.want)
+            res = self.engine.poll(e, view["ref"], self.want)
             if res is not None:
                 self.done[t] = (res, arr, now)
                 del self.pending[t]

@@ -1,38 +1,52 @@
 """MVS throughput benchmark on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1|c2|c3|c4|c5]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c5|c2|c3|c4|c1]
                     [--batched-step auto|1|2] [--tune-steps 1,2] [--no-cpu-baseline]
+                    [--no-nested]
 
-Configurations (synthetic sites, seeded; images resident in HBM before
-timing; one JSON line on stdout):
+A step is one red-black half-step of every reference view of the workload
+(images resident in HBM before timing).  A full solve is 2 x iterations
+half-steps of the photometric pass followed by as many of the geometric-
+consistency pass: with the default 5 + 5 iterations, 20 steps are exactly one
+solve.  The photometric outputs are kept on the device in the step that ends
+that pass, the geometric pass begins in the next step, and the filter, the
+outputs and the 2.5D DSM fusion run in the solve's last step.  W warmup steps
+run the same sequence on a solve that is discarded; the K timed steps start a
+fresh solve (steps beyond 20 start the next one).  views/s = views x K / 20 /
+wall time of the K steps.
 
-c2 (default)  10 views of 1024x1024, every view a reference with 8 sources;
-              photometric pass (5 iterations), geometric pass + filter
-              (5 iterations), 0.5 m DSM fusion.  One step solves and fuses
-              all 10 views.  Two forms on the same inputs:
-                plain    one reference view per solve call, synchronised,
-                         window step 1;
-                batched  all views in one solve per pass, the half-step
-                         sequence replayed from a captured hipGraph, window
-                         step chosen by the autotuner.
-              `value` = batched views/s; plain views/s, per-view latency
-              percentiles and the DSM error of both forms vs the synthetic
-              ground truth are reported beside it.
-c3            32 reference views of ragged crop sizes (512..2048 per side),
-              6..12 sources each, same passes; plain vs batched.
-c4            serving: Poisson arrivals (default 20 req/s) of per-view depth
-              requests from 4 sites (crops 1024..2048 per side, 8..11
-              sources); continuous batching with HBM-footprint admission;
-              p50/p99 latency and views/s, against a one-request-at-a-time
-              queue on the same arrival schedule; `capacity` repeats both
-              forms with every request queued at t = 0.
-c5            40 views of 4096x4096, 20 sources, both passes, 0.3 m DSM.
-c1            2 views of 256x256, 1 source, photometric only (the case the
-              CPU oracle runs in full).
+Two forms on the same inputs:
+  plain    one view per call (vsm_batch_* on a single view, synchronised),
+           window step 1 (every tap, the reference's default schedule); a
+           step runs that half-step for each view in turn;
+  batched  all views in one persistent launch per half-step replayed from a
+           captured hipGraph, window step chosen by the autotuner.
+`value` = batched views/s.  Beside it: plain views/s, per-view latency
+percentiles, the DSM median |error| of both forms against the synthetic
+ground truth, the plain form at the batched form's window step
+(`batched_over_plain_same_step`), the k_halfstep roofline and a CPU baseline.
+
+Configurations (synthetic sites, seeded):
+c5 (default)  40 views of 4096x4096 over a 2 km^2 site with 500 buildings,
+              20 sources each, photometric + geometric 5 + 5 iterations,
+              filter, 0.3 m DSM.  The 10 x 1024^2, 8-source workload (c2) is
+              run after it and reported under "c2" (with the oracle accuracy
+              bar of tests/golden/c2_oracle_dsm.json).
+c2            10 views of 1024x1024, 8 sources, 0.5 m DSM.
+c3            32 views of ragged crop sizes (512..2048 per side), 6..12 sources.
+c4            serving: Poisson arrivals of per-view requests from 4 sites
+              (17 views each, crops 1024..3072 per side, 8..16 sources), each
+              request a photometric pass then the geometric pass with the
+              filter; continuous batching with HBM-footprint admission,
+              p50/p99 latency and views/s, against a one-solve-at-a-time
+              queue on the same schedule; incremental per-site DSM.
+c1            2 views of 256x256, 1 source, photometric only (the case the CPU
+              oracle runs in full).
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -46,10 +60,22 @@ METRIC = "MVS views/s + p50/p99 per-view latency, batched vs plain, 1 MI355X"
 FP32_PEAK_TFLOPS = 157.3      # MI355X vector FP32 (published)
 HBM_PEAK_GBPS = 8000.0        # MI355X HBM3E (published)
 FLOPS_PER_TAP = 30            # homography 12, divide 3, bilinear 8, accumulate 7
+ORACLE_BAR_M = 0.05           # GPU DSM error may exceed the oracle's by at most this
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fp:
+            for line in fp:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
 
 
 # ---------------------------------------------------------------------------
@@ -101,68 +127,144 @@ def dsm_error(dsm, gt):
     return float(np.median(np.abs(dsm[ok] - gt[ok]))), float(ok.mean())
 
 
-# ---------------------------------------------------------------------------
-# plain vs batched
-# ---------------------------------------------------------------------------
-PASSES_FULL = ((0, dict(geom_consistency=0, filter=0)), (1, dict(geom_consistency=1, filter=1)))
-PASSES_PHOTO = ((0, dict(geom_consistency=0, filter=0)),)
+def _sync(ctx):
+    ctx.synchronize()
+    import torch
+    if torch.cuda.is_available():
+        torch.cuda.synchronize()
 
 
-def run_form(ctx, views, grid, form, iters, use_graph, photometric_only=False):
-    """One step: photometric (+ geometric with filter) for every view, then
-    fusion.  Returns (wall seconds, per-view latencies, fused DSM)."""
-    lat = {}
-    t0 = time.perf_counter()
-    for pass_, extra in (PASSES_PHOTO if photometric_only else PASSES_FULL):
-        ctx.set_params(num_iterations=iters, **extra)
-        if form == "plain":
-            for v in views:
-                ts = time.perf_counter()
-                ctx.solve([v], pass_, want=())
-                lat[v["ref"]] = lat.get(v["ref"], 0.0) + time.perf_counter() - ts
+# ---------------------------------------------------------------------------
+# plain vs batched, step by step
+# ---------------------------------------------------------------------------
+PASS_PARAMS = {0: dict(geom_consistency=0, filter=0), 1: dict(geom_consistency=1, filter=1)}
+
+
+class FormRunner:
+    """The step sequence of one form over a set of views (see the module
+    docstring): step k runs half-step k mod S of solve k // S for every view."""
+
+    def __init__(self, ctx, views, form, window_step, iters, grid, photometric_only=False):
+        self.ctx, self.views, self.form = ctx, list(views), form
+        self.window_step, self.iters, self.grid = window_step, iters, grid
+        self.photometric_only = photometric_only
+        self.P = 2 * iters
+        self.S = self.P if photometric_only else 2 * self.P
+        self.open = []                 # open batches (batched: one; plain: one per view)
+        self.view_s = {}               # plain: per-view seconds of the current solve
+        self.t_solve = None
+        self.latencies = []            # per view (plain) / per solve, for every view (batched)
+        self.dsm = None
+        self.solves = 0
+
+    def _params(self, pass_):
+        self.ctx.set_params(num_iterations=self.iters, window_step=self.window_step, **PASS_PARAMS[pass_])
+
+    def _fuse(self):
+        ids = [v["ref"] for v in self.views]
+        if self.photometric_only:      # two views cannot pass the <= 2 rejection: keep every cell
+            return self.ctx.fuse_dsm(self.grid, ids, depth_kind=0, min_measurements=1)
+        return self.ctx.fuse_dsm(self.grid, ids, depth_kind=1, min_measurements=3)
+
+    def step(self, k):
+        j = k % self.S
+        if self.form == "batched":
+            self._step_batched(j)
         else:
-            ctx.solve(views, pass_, want=(), use_graph=use_graph)
-    # two views (c1) cannot pass the reference's <= 2 rejection: keep every cell there
-    dsm = ctx.fuse_dsm(grid, [v["ref"] for v in views], depth_kind=0 if photometric_only else 1,
-                       min_measurements=1 if photometric_only else 3)
-    wall = time.perf_counter() - t0
-    if form != "plain":
-        lat = {v["ref"]: wall for v in views}
-    return wall, list(lat.values()), dsm
+            self._step_plain(j)
+
+    def _step_batched(self, j):
+        c = self.ctx
+        if j == 0:
+            self.t_solve = time.perf_counter()
+            self._params(0)
+            self.open = [c.batch(self.views, 0, use_graph=True)]
+        elif j == self.P:
+            self.open[0].end(want=())          # photometric maps stay on the device
+            self._params(1)
+            self.open = [c.batch(self.views, 1, use_graph=True)]
+        self.open[0].step(1)
+        if j == self.S - 1:
+            self.open.pop().end(want=())
+            lat = time.perf_counter() - self.t_solve
+            self.latencies += [lat] * len(self.views)
+            self.dsm = self._fuse()
+            self.solves += 1
+
+    def _timed(self, v, fn):
+        t = time.perf_counter()
+        r = fn()
+        self.view_s[v["ref"]] += time.perf_counter() - t
+        return r
+
+    def _step_plain(self, j):
+        c = self.ctx
+        if j == 0:
+            self.view_s = {v["ref"]: 0.0 for v in self.views}
+            self._params(0)
+            self.open = [self._timed(v, lambda v=v: c.batch([v], 0)) for v in self.views]
+        elif j == self.P:
+            for v, b in zip(self.views, self.open):
+                self._timed(v, lambda b=b: b.end(want=()))
+            self._params(1)
+            self.open = [self._timed(v, lambda v=v: c.batch([v], 1)) for v in self.views]
+        for v, b in zip(self.views, self.open):
+            self._timed(v, lambda b=b: b.step(1))
+        if j == self.S - 1:
+            for v, b in zip(self.views, self.open):
+                self._timed(v, lambda b=b: b.end(want=()))
+            self.open = []
+            self.latencies += list(self.view_s.values())
+            self.dsm = self._fuse()
+            self.solves += 1
+
+    def discard(self):
+        for b in self.open:
+            b.discard()
+        self.open = []
 
 
-def measure_forms(ctx, views, grid, gt, a, batched_step):
-    res = {}
-    for form in [f for f in ("plain", "batched") if f in a.forms.split(",")]:
-        ctx.set_params(window_step=1 if form == "plain" else batched_step)
-        use_graph = form == "batched"
-        po = getattr(a, "photometric_only", False)
-        for _ in range(a.warmup):
-            run_form(ctx, views, grid, form, a.iters, use_graph, po)
-        log(f"{form}: warm")
-        ctx.reset_stats()
-        ctx.set_timing(form == "batched")
-        walls, lats = [], []
-        ctx.synchronize()
-        for k in range(a.steps):
-            w, lat, dsm = run_form(ctx, views, grid, form, a.iters, use_graph, po)
-            walls.append(w)
-            lats += lat
-            log(f"{form} step {k}: {w * 1e3:.0f} ms")
-        ctx.synchronize()
-        st = ctx.stats()
-        ctx.set_timing(False)
-        err, cov = dsm_error(dsm, gt)
-        res[form] = dict(ms_per_step=1e3 * float(np.mean(walls)),
-                         views_per_s=len(views) * a.steps / float(np.sum(walls)),
-                         p50_ms=1e3 * float(np.percentile(lats, 50)),
-                         p99_ms=1e3 * float(np.percentile(lats, 99)),
-                         dsm_err=err, dsm_cov=cov, stats=st)
-        log(form, {k: v for k, v in res[form].items() if k != "stats"})
-    return res
+def run_form(ctx, views, form, window_step, a, grid, steps, warmup, photometric_only=False,
+             timing=True):
+    """Warmup steps on a discarded solve, then `steps` timed steps from a
+    fresh solve.  Returns per-step walls, the runner and the device stats of
+    the timed steps."""
+    r = FormRunner(ctx, views, form, window_step, a.iters, grid, photometric_only)
+    for k in range(warmup):
+        r.step(k)
+    r.discard()
+    r = FormRunner(ctx, views, form, window_step, a.iters, grid, photometric_only)
+    ctx.reset_stats()
+    ctx.set_timing(timing)
+    walls = []
+    _sync(ctx)
+    for k in range(steps):
+        t = time.perf_counter()
+        r.step(k)
+        walls.append(time.perf_counter() - t)
+        if k % 5 == 4 or k == steps - 1:
+            log(f"{form} (window step {window_step}) step {k}: {1e3 * walls[-1]:.0f} ms")
+    _sync(ctx)
+    st = ctx.stats()
+    ctx.set_timing(False)
+    r.discard()
+    return walls, r, st
 
 
-def roofline(st, cfg="c2"):
+def summarize(walls, r, n_views, gt):
+    tot = float(np.sum(walls))
+    out = dict(ms_per_step=1e3 * float(np.mean(walls)),
+               views_per_s=n_views * len(walls) / r.S / tot, wall_s=tot, solves=r.solves,
+               step_ms=[round(1e3 * w, 2) for w in walls])
+    if r.latencies:
+        out["p50_ms"] = 1e3 * float(np.percentile(r.latencies, 50))
+        out["p99_ms"] = 1e3 * float(np.percentile(r.latencies, 99))
+    if r.dsm is not None:
+        out["dsm_err"], out["dsm_cov"] = dsm_error(r.dsm, gt)
+    return out
+
+
+def roofline(st, cfg):
     if not (st["halfstep_ms"] > 0 and st["halfstep_launches"] > 0):
         return None
     avg_ms = st["halfstep_ms"] / st["halfstep_launches"]
@@ -170,16 +272,14 @@ def roofline(st, cfg="c2"):
     ach = FLOPS_PER_TAP * taps / (avg_ms * 1e-3) / 1e12
     traffic, source, profiled = None, None, None
     # HBM traffic per launch comes from a committed counter profile of the
-    # same configuration (none for the others)
+    # same configuration and kernel
     tfile = os.path.join(ROOT, "profiles", f"k_halfstep_{cfg}_counters.json")
     if os.path.exists(tfile):
         t = json.load(open(tfile))
         traffic = round(t["fetch_bytes"] + t["write_bytes"])
         source = (f"profiles/k_halfstep_{cfg}_counters.md: FETCH_SIZE + WRITE_SIZE per dispatch, "
-                  "calibrated on an elementwise kernel of known bytes (batched, window step 2)"
+                  "calibrated on an elementwise kernel of known bytes (batched form)"
                   + (f"; {t['calibration']}" if "calibration" in t else ""))
-        # the profile's traffic over this run's live launch time, against the
-        # 8 TB/s HBM peak; VALU issue and lane figures as the profile read them
         gbps = traffic / (avg_ms * 1e-3) / 1e9
         profiled = {"hbm_gbps": round(gbps, 1), "hbm_frac": round(gbps / HBM_PEAK_GBPS, 4),
                     "valu_busy_pct": round(t["valubusy"], 1) if "valubusy" in t else None,
@@ -191,13 +291,16 @@ def roofline(st, cfg="c2"):
             "bound_note": "vector FP32 (VALU): a gather/stencil kernel, neither MFMA nor HBM "
                           "work; peak = MI355X FP32 157.3 TFLOP/s",
             "avg_launch_ms": round(avg_ms, 4), "taps_per_launch": int(taps),
-            "flops_per_tap": FLOPS_PER_TAP}
+            "flops_per_tap": FLOPS_PER_TAP, "timing": "HIP events on the context stream around "
+                                                      "every k_halfstep launch of the timed steps"}
 
 
-def cpu_baseline(site, window_radius, iters):
-    """Oracle (FP64 C, OpenMP) on a bounded sample: a 512x512 crop of the
-    first reference view with all its sources, 1 photometric iteration,
-    extrapolated to a full view (area x iterations of both passes)."""
+def cpu_baseline(ctx, site, window_radius, iters, crop=512):
+    """Oracle (FP64 C, OpenMP) on a bounded sample: a crop of the first
+    reference view with all its sources, one photometric iteration, then one
+    geometric iteration from that state against the sources' photometric
+    depth maps the GPU left on the device; extrapolated to a full view (area)
+    and to the workload's iterations of each pass."""
     import pm_oracle as po
     v = site["views"][0]
     inv_ids = {i: n for n, i in site["ids"].items()}
@@ -205,23 +308,30 @@ def cpu_baseline(site, window_radius, iters):
     srcs = [inv_ids[s] for s in v["srcs"]]
     imgs, P4 = site["imgs"], site["P4"]
     H, W = imgs[ref].shape
-    ch, cw = min(512, H), min(512, W)
+    ch, cw = min(crop, H), min(crop, W)
     y0, x0 = (H - ch) // 2, (W - cw) // 2
-    crop = np.ascontiguousarray(imgs[ref][y0:y0 + ch, x0:x0 + cw])
+    crop_img = np.ascontiguousarray(imgs[ref][y0:y0 + ch, x0:x0 + cw])
     T = np.array([[1, 0, x0, 0], [0, 1, y0, 0], [0, 0, 1, 0], [0, 0, 0, 1]], dtype=np.float64)
     P4c = np.linalg.inv(T) @ P4[ref]          # pixel (c, r) of the crop is (c + x0, r + y0)
-    ov = po.OracleView(crop, [imgs[s] for s in srcs], P4c, [P4[s] for s in srcs], v["depth_range"],
-                       view_id=v["ref"], params=dict(window_radius=window_radius))
+    src_depths = [ctx.get_depth_map(s, kind=0) for s in v["srcs"]]
+    ov = po.OracleView(crop_img, [imgs[s] for s in srcs], P4c, [P4[s] for s in srcs], v["depth_range"],
+                       view_id=v["ref"], src_depths=src_depths,
+                       params=dict(window_radius=window_radius))
     t = time.perf_counter()
-    ov.solve(0, 1)
-    dt = time.perf_counter() - t
+    st = ov.solve(0, 1)
+    t_photo = time.perf_counter() - t
+    t = time.perf_counter()
+    ov.solve(1, 1, state=st, do_filter=True)
+    t_geo = time.perf_counter() - t
     scale = (H * W) / (ch * cw)
-    per_view = dt * scale * (2 * iters)
+    per_view = scale * iters * (t_photo + t_geo)
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": round(1.0 / per_view, 6), "unit": "views/s", "cores": cores, "kind": "port",
-            "sample": f"{ch}x{cw} crop of one {W}x{H} reference view, {len(srcs)} sources, "
-                      f"1 photometric iteration of the FP64 oracle in {dt:.1f}s; per-view time = "
-                      f"sample x {scale:.0f} (area) x {2 * iters} (iterations of both passes)"}
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "sample": f"{ch}x{cw} crop of one {W}x{H} reference view, {len(srcs)} sources, window "
+                      f"step 1: 1 photometric iteration of the FP64 oracle in {t_photo:.1f}s, then 1 "
+                      f"geometric iteration + filter in {t_geo:.1f}s; per-view time = (sum) x "
+                      f"{scale:.0f} (area) x {iters} (iterations per pass)"}
 
 
 def cpu_baseline_full(site, window_radius, iters):
@@ -241,14 +351,15 @@ def cpu_baseline_full(site, window_radius, iters):
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     n = len(site["views"])
     return {"value": round(n / dt, 4), "unit": "views/s", "cores": cores, "kind": "port",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
             "sample": f"the whole workload: {n} views, {iters} photometric iterations, in {dt:.2f}s"}
 
 
 def cpu_baseline_fusion(ctx, site, grid, max_views=2, band_rows=256):
     """numpy restatement of aggregate_2p5d (oracle/fusion_oracle.py), one
-    process, on the geometric depth maps of the last batched step: per-view
-    DSMs timed on up to max_views views and scaled to all of them, the
-    cross-view fusion timed on a band of band_rows grid rows and scaled to
+    process, on the geometric depth maps the batched form left on the device:
+    per-view DSMs timed on up to max_views views and scaled to all of them,
+    the cross-view fusion timed on a band of band_rows grid rows and scaled to
     the whole grid (the numpy median of big stacks runs for minutes)."""
     import fusion_oracle as fo
     ids = [v["ref"] for v in site["views"]]
@@ -257,7 +368,6 @@ def cpu_baseline_fusion(ctx, site, grid, max_views=2, band_rows=256):
     per_view = []
     for i, n in zip(ids[:k], site["names"][:k]):
         per_view.append(fo.view_dsm(ctx.get_depth_map(i, kind=1), np.linalg.inv(site["P4"][n]), grid))
-        log(f"cpu fusion baseline: per-view DSM {n} done")
     t_view = (time.perf_counter() - t) / k
     rows = min(band_rows, grid[5])
     band = np.stack([pv[:rows] for pv in per_view] * (len(ids) // k + 1))[:len(ids)]
@@ -294,18 +404,17 @@ def choose_step(ctx, views, a):
     return st, rep
 
 
-def bench_forms(a, cfg):
-    from vissatsatellitestereo_amd import synth, runtime
-    res_m = 0.5
-    a.photometric_only = cfg == "c1"
+def workload(cfg):
+    from vissatsatellitestereo_amd import synth
+    res_m, counts = 0.5, None
     if cfg == "c1":
-        spec, max_src, counts = synth.spec_c1(), 1, None
+        spec, max_src = synth.spec_c1(), 1
         desc = "2 views x 256x256 uint8, 1 source each, photometric only"
     elif cfg == "c2":
-        spec, max_src, counts = synth.spec_c2(), 8, None
+        spec, max_src = synth.spec_c2(), 8
         desc = "10 reference views x 1024x1024 uint8, 8 sources"
     elif cfg == "c5":
-        spec, max_src, counts = synth.spec_c5(), 20, None
+        spec, max_src = synth.spec_c5(), 20
         desc = "40 reference views x 4096x4096 uint8, 20 sources, 500 buildings"
         res_m = 0.3
     else:
@@ -314,6 +423,20 @@ def bench_forms(a, cfg):
         counts = [int(k) for k in rng.integers(6, 13, spec.n_views)]
         max_src = 12
         desc = "32 reference views, ragged crops 512..2048 per side, 6..12 sources"
+    return spec, max_src, counts, res_m, desc
+
+
+def oracle_bar(cfg):
+    f = os.path.join(ROOT, "tests", "golden", f"{cfg}_oracle_dsm.json")
+    if not os.path.exists(f):
+        return None
+    return json.load(open(f))
+
+
+def bench_forms(a, cfg, steps, warmup, with_cpu=True):
+    from vissatsatellitestereo_amd import runtime
+    photometric_only = cfg == "c1"
+    spec, max_src, counts, res_m, desc = workload(cfg)
     work = build_site(spec, os.path.join(a.work, cfg), max_src, counts, res_m)
     if a.build_only:
         return None
@@ -324,66 +447,120 @@ def bench_forms(a, cfg):
     ctx = make_ctx([site], a)
     views = site["views"]
     step, tune_rep = choose_step(ctx, views, a)
-    res = measure_forms(ctx, views, grid, gt, a, step)
+    res = {}
+    for form in [f for f in ("batched", "plain") if f in a.forms.split(",")]:
+        ws = 1 if form == "plain" else step
+        walls, r, st = run_form(ctx, views, form, ws, a, grid, steps, warmup, photometric_only)
+        res[form] = summarize(walls, r, len(views), gt)
+        res[form]["stats"] = st
+        log(form, {k: v for k, v in res[form].items() if k not in ("stats", "step_ms")})
     b = res.get("batched") or res["plain"]
     p = res.get("plain") or b
-    if a.no_cpu_baseline:
-        cpu = cpu_fuse = None
-    elif cfg == "c1":
-        cpu = cpu_baseline_full(site, a.window_radius, a.iters)
-        cpu_fuse = None
-    else:
-        log("cpu baseline: PatchMatch oracle on a crop")
-        cpu = cpu_baseline(site, a.window_radius, a.iters)
-        log("cpu baseline: fusion")
-        cpu_fuse = cpu_baseline_fusion(ctx, site, grid)
+    # the plain form at the batched form's window step: what batching adds
+    # without the stride.  On c5 over the first photometric half-steps only
+    # (the timed region of the full leg takes ~100 s).
+    same = None
+    if "plain" in res and "batched" in res and step != 1 and not a.no_same_step:
+        m = steps if cfg != "c5" else min(steps, a.same_step_steps)
+        walls, r, _ = run_form(ctx, views, "plain", step, a, grid, m, min(warmup, 1), photometric_only,
+                               timing=False)
+        t_plain = float(np.sum(walls))
+        t_batched = float(np.sum(b["step_ms"][:m])) / 1e3
+        same = {"window_step": step, "steps": m,
+                "plain_views_per_s": round(len(views) * m / r.S / t_plain, 4),
+                "batched_over_plain": round(t_plain / t_batched, 3),
+                "note": ("both forms on the first %d steps of a solve" % m) if m < steps
+                        else "both forms on the whole timed region"}
+        log("same window step", same)
+    cpu = cpu_fuse = None
+    if with_cpu and not a.no_cpu_baseline and "batched" in res:
+        if cfg == "c1":
+            cpu = cpu_baseline_full(site, a.window_radius, a.iters)
+        else:
+            # the sample reads the sources' photometric maps and the fusion
+            # baseline the geometric maps of the last complete solve; with
+            # fewer timed steps than one solve they come from one more batched
+            # solve (outside every timed region)
+            from vissatsatellitestereo_amd._lib import VsmError
+            try:
+                for v in views:
+                    ctx.get_depth_map(v["ref"], kind=1)
+            except VsmError:
+                for pass_ in (0, 1):
+                    ctx.set_params(num_iterations=a.iters, window_step=step, **PASS_PARAMS[pass_])
+                    ctx.solve(views, pass_, want=(), use_graph=True)
+            log("cpu baseline: PatchMatch oracle on a crop")
+            cpu = cpu_baseline(ctx, site, a.window_radius, a.iters, crop=a.cpu_crop)
+            log("cpu baseline: fusion")
+            cpu_fuse = cpu_baseline_fusion(ctx, site, grid)
+    ctx.close()
+    errs = {"plain": p.get("dsm_err"), "batched": b.get("dsm_err"),
+            "coverage_plain": round(p.get("dsm_cov", 0.0), 4),
+            "coverage_batched": round(b.get("dsm_cov", 0.0), 4)}
+    bar = oracle_bar(cfg)
+    if bar is not None and errs["plain"] is not None and errs["batched"] is not None:
+        lim = bar["dsm_median_abs_err_m"] + ORACLE_BAR_M
+        errs["oracle"] = bar["dsm_median_abs_err_m"]
+        errs["bar"] = round(lim, 5)
+        errs["within_bar"] = bool(errs["plain"] <= lim and errs["batched"] <= lim)
+        errs["oracle_source"] = "tests/golden/%s_oracle_dsm.json (%s)" % (cfg, bar.get("solver", ""))
+    S = 2 * a.iters * (1 if photometric_only else 2)
     out = {
         "metric": METRIC, "value": round(b["views_per_s"], 4), "unit": "views/s", "n_gpus": 1,
-        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(b["ms_per_step"], 2),
+        "steps": steps, "warmup": warmup, "ms_per_step": round(b["ms_per_step"], 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": f"{cfg}: {desc}, photometric+geometric {a.iters}+{a.iters} "
-                               f"iterations, filter, r={a.window_radius}, {res_m} m DSM fusion",
-                   "forms": {"plain": "one view per solve call, window step 1",
-                             "batched": f"all views per solve call, persistent kernel, hipGraph "
-                                        f"replay, window step {step} (autotuner)"},
+        "config": {"workload": f"{cfg}: {desc}, photometric+geometric {a.iters}+{a.iters} iterations, "
+                               f"filter, r={a.window_radius}, {res_m} m DSM fusion; a step is one "
+                               f"half-step of every view, {S} steps one solve",
+                   "forms": {"plain": "one view per call, window step 1",
+                             "batched": f"all views per launch, persistent kernel, hipGraph replay, "
+                                        f"window step {step} (autotuner)"},
                    "autotune": tune_rep},
         "plain_views_per_s": round(p["views_per_s"], 4),
         "batched_over_plain": round(b["views_per_s"] / p["views_per_s"], 3),
-        "latency_ms": {"plain_p50": round(p["p50_ms"], 1), "plain_p99": round(p["p99_ms"], 1),
-                       "batched_p50": round(b["p50_ms"], 1), "batched_p99": round(b["p99_ms"], 1)},
-        "dsm_median_abs_err_m": {"plain": p["dsm_err"], "batched": b["dsm_err"],
-                                 "coverage_plain": round(p["dsm_cov"], 4),
-                                 "coverage_batched": round(b["dsm_cov"], 4)},
+        "batched_over_plain_same_step": same,
+        "latency_ms": {"plain_p50": round(p.get("p50_ms", float("nan")), 1),
+                       "plain_p99": round(p.get("p99_ms", float("nan")), 1),
+                       "batched_p50": round(b.get("p50_ms", float("nan")), 1),
+                       "batched_p99": round(b.get("p99_ms", float("nan")), 1)},
+        "dsm_median_abs_err_m": errs,
         "roofline": roofline(b["stats"], cfg),
         "cpu_baseline": cpu,
-        "fusion": {"gpu_ms_per_step": round(b["stats"]["fuse_ms"] / max(1, a.steps), 3)
-                   if b["stats"]["fuse_ms"] else None,
+        "fusion": {"gpu_ms_per_solve": round(b["stats"]["fuse_ms"] / max(1, res["batched"]["solves"]), 3)
+                   if "batched" in res and b["stats"]["fuse_ms"] else None,
                    "cpu_baseline": cpu_fuse},
+        "step_ms": {"batched": b["step_ms"], "plain": p["step_ms"]},
     }
-    if a.photometric_only:
+    if photometric_only:
         out["config"]["workload"] = (f"{cfg}: {desc}, {a.iters} iterations, r={a.window_radius}, "
-                                     f"{res_m} m DSM fusion of the photometric maps (every cell kept)")
-    ctx.close()
+                                     f"{res_m} m DSM fusion of the photometric maps (every cell kept); "
+                                     f"a step is one half-step of every view, {S} steps one solve")
     return out
 
 
 # ---------------------------------------------------------------------------
 # serving
 # ---------------------------------------------------------------------------
-def bench_serving(a):
+def serving_sites(a):
     from vissatsatellitestereo_amd import synth
-    from vissatsatellitestereo_amd.serving import ServeLoop, poisson_arrivals
     works = []
     for k in range(4):
-        spec = synth.spec_ragged(n_views=12, lo=1024, hi=a.serve_max_side, seed=50 + k, n_boxes=30,
-                                 seed_offset=10 * (k + 1))
+        spec = synth.spec_ragged(n_views=a.serve_views, lo=1024, hi=a.serve_max_side, seed=50 + k,
+                                 n_boxes=30, seed_offset=10 * (k + 1))
         rng = np.random.default_rng(60 + k)
-        counts = [int(c) for c in rng.integers(8, 12, spec.n_views)]
-        works.append(build_site(spec, os.path.join(a.work, f"c4_site{k}"), 11, counts))
+        counts = [int(c) for c in rng.integers(8, min(16, a.serve_views - 1) + 1, spec.n_views)]
+        works.append(build_site(spec, os.path.join(a.work, f"c4_site{k}_{a.serve_views}_{a.serve_max_side}"),
+                                16, counts))
+    return works
+
+
+def bench_serving(a):
+    from vissatsatellitestereo_amd import runtime
+    from vissatsatellitestereo_amd.serving import ServeLoop, poisson_arrivals
+    works = serving_sites(a)
     if a.build_only:
         return None
-    from vissatsatellitestereo_amd import runtime
     sites = [load_site(w, id_base=1000 * (k + 1)) for k, w in enumerate(works)]
     grids = [runtime.dsm_grid(runtime.read_aoi(w)) for w in works]
     gts = [np.load(os.path.join(w, "synthetic_gt", "gt_dsm.npy")) for w in works]
@@ -391,15 +568,18 @@ def bench_serving(a):
     copy_calibration(a)
     ctx = make_ctx(sites, a)
     views = [v for s in sites for v in s["views"]]
-    ctx.set_params(num_iterations=a.iters, geom_consistency=0, filter=1)
     step, tune_rep = choose_step(ctx, views, a)
     rng = np.random.default_rng(6)
     order = rng.permutation(len(views))
-    reqs = [views[i] for i in order][:a.requests]
+    # every view of every site is requested once (a view's geometric pass
+    # needs the photometric results of its sources, which are views of its site)
+    reqs = [views[i] for i in order]
     arrivals = poisson_arrivals(len(reqs), a.rate, seed=6)
+    n_src = [len(v["srcs"]) for v in reqs]
+    sizes = [ctx.sizes[v["ref"]] for v in reqs]
 
     # incremental per-site DSM: each finished request re-fuses its site from
-    # the photometric (filtered) depth maps of the site's finished requests
+    # the geometric depth maps of the site's finished requests
     fused = {"updates": 0, "ms": 0.0}
     finished = {k: [] for k in range(len(sites))}
     dsm = {}
@@ -409,7 +589,7 @@ def bench_serving(a):
         finished[k].append(view["ref"])
         if len(finished[k]) >= 3:
             t = time.perf_counter()
-            dsm[k] = ctx.fuse_dsm(grids[k], finished[k], depth_kind=0, min_measurements=3)
+            dsm[k] = ctx.fuse_dsm(grids[k], finished[k], depth_kind=1, min_measurements=3)
             fused["ms"] += 1e3 * (time.perf_counter() - t)
             fused["updates"] += 1
 
@@ -418,77 +598,100 @@ def bench_serving(a):
         errs = [e for e in errs if e is not None]
         return round(float(np.median(errs)), 4) if errs else None
 
-    ctx.set_params(num_iterations=a.iters, geom_consistency=0, filter=1, window_step=step)
-    warm = ServeLoop(ctx)                    # one request end to end
-    warm.submit(reqs[0])
+    ctx.set_params(num_iterations=a.iters, window_step=step, **PASS_PARAMS[1])
+    warm = ServeLoop(ctx)                              # two photometric requests end to end
+    for v in sites[0]["views"][:2]:
+        warm.submit(v)
     warm.drain()
-    loop = ServeLoop(ctx, on_complete=on_complete)
+    loop = ServeLoop(ctx, on_complete=on_complete, geometric=True)
     ctx.reset_stats()
     served = loop.run_schedule(arrivals, reqs)
     st = ctx.stats()
     served_err = site_errors()
     log("served", {k: v for k, v in served.items() if k not in ("tickets", "latency_s")},
         "dsm", served_err, fused)
-    # one-request-at-a-time queue on the same arrival schedule (plain form)
+    # plain: one solve at a time in arrival order, each a single view at
+    # window step 1; a view's geometric solve runs once the photometric
+    # solves of the view and its sources are done
     ctx.set_params(window_step=1)
-    t0 = time.perf_counter()
-    lat = []
-    for t_arr, v in zip(arrivals, reqs):
-        now = time.perf_counter() - t0
-        if now < t_arr:
-            time.sleep(t_arr - now)
-        ctx.solve([v], 0, want=("depth",))
-        lat.append(time.perf_counter() - t0 - t_arr)
-    plain_wall = time.perf_counter() - t0
+    plain = queue_schedule(ctx, arrivals, reqs)
     for k in list(dsm):
-        dsm[k] = ctx.fuse_dsm(grids[k], finished[k], depth_kind=0, min_measurements=3)
+        dsm[k] = ctx.fuse_dsm(grids[k], finished[k], depth_kind=1, min_measurements=3)
     queue_err = site_errors()
-    log(f"queue: {len(reqs) / plain_wall:.2f} views/s", "dsm", queue_err)
-    # capacity: every request queued at t = 0, so neither form waits for
-    # arrivals (at the configured rate the served form keeps up with the
-    # arrivals and its views/s is the offered load)
+    log(f"queue: {plain['views_per_s']:.2f} views/s", "dsm", queue_err)
+    # capacity: every request queued at t = 0 (at the configured rate the
+    # served form keeps up with the arrivals and its views/s is the offered load)
     ctx.set_params(window_step=step)
-    burst = ServeLoop(ctx).run_schedule([0.0] * len(reqs), reqs)
+    burst = ServeLoop(ctx, geometric=True).run_schedule([0.0] * len(reqs), reqs)
     ctx.set_params(window_step=1)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for v in reqs:
-        ctx.solve([v], 0, want=("depth",))
-    plain_cap = len(reqs) / (time.perf_counter() - t0)
-    log(f"capacity: served {burst['views_per_s']:.2f} views/s, plain {plain_cap:.2f} views/s")
+    plain_cap = queue_schedule(ctx, [0.0] * len(reqs), reqs)
+    log(f"capacity: served {burst['views_per_s']:.2f} views/s, plain {plain_cap['views_per_s']:.2f} views/s")
     out = {
         "metric": METRIC, "value": round(served["views_per_s"], 4), "unit": "views/s", "n_gpus": 1,
         "steps": len(reqs), "warmup": 1, "ms_per_step": round(1e3 * served["wall_s"] / len(reqs), 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": f"c4 serving: {len(reqs)} per-view requests, Poisson {a.rate} req/s, "
-                               f"4 sites, crops 1024..{a.serve_max_side} per side, 8..11 sources, "
-                               f"photometric {a.iters} iterations + filter, r={a.window_radius}, "
-                               f"HBM budget {a.hbm_budget_gb} GB, incremental per-site 0.5 m DSM",
-                   "forms": {"plain": "one request at a time in arrival order, window step 1",
-                             "batched": f"continuous batching (vsm_submit / vsm_serve_step), "
+                               f"4 sites x {a.serve_views} views, crops 1024..{a.serve_max_side} per side "
+                               f"(largest {max(max(s) for s in sizes)}), {min(n_src)}..{max(n_src)} "
+                               f"sources; each request photometric {a.iters} iterations then geometric "
+                               f"{a.iters} iterations + filter, r={a.window_radius}, HBM budget "
+                               f"{a.hbm_budget_gb} GB, incremental per-site 0.5 m DSM of the geometric maps",
+                   "forms": {"plain": "one solve at a time in arrival order, window step 1",
+                             "batched": f"continuous batching (vsm_submit_pass / vsm_serve_step), "
                                         f"window step {step} (autotuner)"},
                    "autotune": tune_rep},
         "dsm_median_abs_err_m": {"plain": queue_err, "batched": served_err,
                                  "note": "median over sites of each site's median |dh|"},
         "incremental_fusion": {"updates": fused["updates"], "ms_total": round(fused["ms"], 1)},
         "latency_ms": {"served_p50": round(served["p50_ms"], 1), "served_p99": round(served["p99_ms"], 1),
-                       "queue_p50": round(1e3 * float(np.percentile(lat, 50)), 1),
-                       "queue_p99": round(1e3 * float(np.percentile(lat, 99)), 1)},
-        "plain_views_per_s": round(len(reqs) / plain_wall, 4),
-        "batched_over_plain": round(served["views_per_s"] / (len(reqs) / plain_wall), 3),
+                       "queue_p50": round(plain["p50_ms"], 1), "queue_p99": round(plain["p99_ms"], 1)},
+        "plain_views_per_s": round(plain["views_per_s"], 4),
+        "batched_over_plain": round(served["views_per_s"] / plain["views_per_s"], 3),
         "capacity": {"note": "all requests queued at t = 0",
                      "served_views_per_s": round(burst["views_per_s"], 4),
-                     "plain_views_per_s": round(plain_cap, 4),
-                     "served_over_plain": round(burst["views_per_s"] / plain_cap, 3),
+                     "plain_views_per_s": round(plain_cap["views_per_s"], 4),
+                     "served_over_plain": round(burst["views_per_s"] / plain_cap["views_per_s"], 3),
                      "served_p50_ms": round(burst["p50_ms"], 1),
                      "served_p99_ms": round(burst["p99_ms"], 1)},
         "serve_steps": served["steps"],
+        "hbm_peak_gb": round(st["hbm_bytes_peak"] / 1e9, 2),
         "roofline": roofline(st, "c4"),
         "cpu_baseline": None,
     }
     ctx.close()
     return out
+
+
+def queue_schedule(ctx, arrivals, reqs):
+    """One solve at a time in arrival order: a view's photometric solve when
+    it arrives, its geometric solve (with the filter) as soon as the
+    photometric solves of the view and all its sources are done.  Latency of
+    a request = its geometric solve's end - its arrival."""
+    order = np.argsort(arrivals)
+    photo_done, geo_done = set(), {}
+    t0 = time.perf_counter()
+    waiting = []
+    for i in order:
+        v = reqs[int(i)]
+        now = time.perf_counter() - t0
+        if now < arrivals[i]:
+            time.sleep(arrivals[i] - now)
+        ctx.set_params(**PASS_PARAMS[0])
+        ctx.solve([v], 0, want=())
+        photo_done.add(v["ref"])
+        waiting.append(int(i))
+        for j in list(waiting):
+            w = reqs[j]
+            if w["ref"] in photo_done and all(s in photo_done for s in w["srcs"]):
+                ctx.set_params(**PASS_PARAMS[1])
+                ctx.solve([w], 1, want=())
+                geo_done[j] = time.perf_counter() - t0
+                waiting.remove(j)
+    wall = time.perf_counter() - t0
+    lat = [geo_done[j] - arrivals[j] for j in geo_done]
+    return dict(views_per_s=len(geo_done) / wall, wall_s=wall, completed=len(geo_done),
+                p50_ms=1e3 * float(np.percentile(lat, 50)), p99_ms=1e3 * float(np.percentile(lat, 99)))
 
 
 def copy_calibration(a):
@@ -512,9 +715,10 @@ def copy_calibration(a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=("c1", "c2", "c3", "c4", "c5"))
+    ap.add_argument("--steps", type=int, default=20, help="timed steps (a step = one half-step of "
+                                                          "every view; 20 = one 5+5 solve)")
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="c5", choices=("c1", "c2", "c3", "c4", "c5"))
     ap.add_argument("--iters", type=int, default=5)
     ap.add_argument("--window-radius", type=int, default=5)
     ap.add_argument("--batched-step", default="auto", help="auto (autotuner) or a fixed window step")
@@ -523,20 +727,36 @@ def main():
                          "allows strides 1 and 2; wider ones only on request)")
     ap.add_argument("--no-tune-cache", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-same-step", action="store_true")
+    ap.add_argument("--same-step-steps", type=int, default=4,
+                    help="c5: steps of the plain form at the batched window step")
+    ap.add_argument("--no-nested", action="store_true", help="c5: skip the nested c2 workload")
+    ap.add_argument("--cpu-crop", type=int, default=512)
     ap.add_argument("--forms", default="plain,batched", help="comma list of plain,batched")
     ap.add_argument("--build-only", action="store_true", help="build the synthetic site(s) and exit")
     ap.add_argument("--copy-calibration-mb", type=int, default=0)
     ap.add_argument("--hbm-budget-gb", type=float, default=200.0)
     ap.add_argument("--rate", type=float, default=20.0, help="c4 arrival rate (req/s)")
-    ap.add_argument("--requests", type=int, default=40, help="c4 number of requests")
-    ap.add_argument("--serve-max-side", type=int, default=2048)
+    ap.add_argument("--serve-views", type=int, default=17, help="c4 views per site")
+    ap.add_argument("--serve-max-side", type=int, default=3072)
     ap.add_argument("--work", default=os.environ.get("VSM_BENCH_WORK", "/tmp/vsm_bench"))
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; runs on more GPUs are not part of it",
                           "n_gpus": a.gpus}))
         sys.exit(2)
-    out = bench_serving(a) if a.config == "c4" else bench_forms(a, a.config)
+    if a.config == "c4":
+        out = bench_serving(a)
+    else:
+        out = bench_forms(a, a.config, a.steps, a.warmup)
+        if out is not None and a.config == "c5" and not a.no_nested:
+            log("nested workload: c2")
+            nested = bench_forms(a, "c2", 20, min(a.warmup, 5), with_cpu=False)
+            keep = ("value", "ms_per_step", "plain_views_per_s", "batched_over_plain",
+                    "batched_over_plain_same_step", "latency_ms", "dsm_median_abs_err_m", "roofline")
+            out["c2"] = {k: nested[k] for k in keep}
+            out["c2"]["workload"] = nested["config"]["workload"]
+            out["c2"]["window_step"] = nested["config"]["autotune"].get("window_step")
     if out is not None:
         print(json.dumps(out))
 

@@ -71,6 +71,12 @@ typedef struct {
     float perturb_decay;              /* per-iteration shrink of perturbations */
     int32_t n_far;                    /* propagation candidates per direction */
     uint64_t seed;
+    /* Counter word of the Monte-Carlo source draws: 0 = the pixel (every
+     * pixel draws independently), 1 = its group of 32 x 4 pixels (columns
+     * 32i..32i+31, rows 4j..4j+3: each pixel still draws from its own
+     * weights, with the uniforms its group shares, so neighbouring pixels
+     * with similar weights draw the same sources). */
+    int32_t draw_group;
 } vsm_params;
 
 /* One reference view to solve. */

@@ -69,6 +69,7 @@ typedef struct {
     double perturb_decay;          /* per iteration */
     int32_t n_far;                 /* candidates per direction */
     uint64_t seed;
+    int32_t draw_group;            /* 1: source draws keyed by 32x4-pixel group, 0: by pixel */
 } orc_params;
 
 typedef struct {
@@ -339,6 +340,18 @@ void orc_reset_state(const orc_view *v, double *cost, double *prob)
     for (size_t i = 0; i < HW * v->n_src; ++i) prob[i] = 0.5;
 }
 
+/* Counter word of a pixel's Monte-Carlo draws: the pixel index, or with
+ * draw_group the index of its group of 32 x 4 pixels (columns 32i..32i+31,
+ * rows 4j..4j+3), so every pixel of a group uses the same uniforms (each
+ * against its own weights). */
+static inline uint32_t draw_key(const orc_view *v, const orc_params *p, uint32_t pixel)
+{
+    if (!p->draw_group) return pixel;
+    const uint32_t W = (uint32_t)v->width;
+    const uint32_t x = pixel % W, y = pixel / W;
+    return (y >> 2) * ((W + 31u) >> 5) + (x >> 5);
+}
+
 /* Monte-Carlo view selection: num_samples draws of a source with probability
  * q[s] / tot (inverse CDF over the sources in index order; a draw that
  * rounding leaves past the last boundary goes to the last source of positive
@@ -355,9 +368,10 @@ static void draw_sources(const orc_view *v, const orc_params *p, uint32_t pixel,
     }
     int last_pos = 0;
     for (int s = 0; s < S; ++s) if (q[s] > 0.0) last_pos = s;
+    const uint32_t key = draw_key(v, p, pixel);
     for (int k = 0; k < p->num_samples; ++k) {
         double ub[4];
-        rng_block(v, p, pixel, code, 2u + (uint32_t)(k >> 2), ub);
+        rng_block(v, p, key, code, 2u + (uint32_t)(k >> 2), ub);
         const double target = ub[k & 3] * tot;
         double acc = 0.0;
         int pick = -1;

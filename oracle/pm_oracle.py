@@ -70,7 +70,7 @@ class OrcParams(ctypes.Structure):
                 ("depth_min", ctypes.c_double), ("depth_max", ctypes.c_double),
                 ("perturbation", ctypes.c_double), ("normal_perturbation", ctypes.c_double),
                 ("max_normal_angle", ctypes.c_double), ("perturb_decay", ctypes.c_double),
-                ("n_far", ctypes.c_int32), ("seed", ctypes.c_uint64)]
+                ("n_far", ctypes.c_int32), ("seed", ctypes.c_uint64), ("draw_group", ctypes.c_int32)]
 
 
 class OrcView(ctypes.Structure):
@@ -96,7 +96,7 @@ def default_params(**kw):
              filter_min_triangulation_angle=4.99, filter_geom_consistency_max_cost=1.0,
              filter_min_num_consistent=2, depth_min=0.0, depth_max=1.0, perturbation=0.05,
              normal_perturbation=0.3, max_normal_angle=80.0, perturb_decay=0.5, n_far=6,
-             seed=0x5A7E11173D)
+             seed=0x5A7E11173D, draw_group=0)
     p.update(kw)
     return p
 
@@ -111,7 +111,7 @@ def to_orc_params(p):
                      p["filter_geom_consistency_max_cost"], p["filter_min_num_consistent"],
                      p["depth_min"], p["depth_max"], p["perturbation"], p["normal_perturbation"],
                      math.radians(p["max_normal_angle"]), p["perturb_decay"], p["n_far"],
-                     p["seed"])
+                     p["seed"], int(p.get("draw_group", 0)))
 
 
 def camera_center(P4):

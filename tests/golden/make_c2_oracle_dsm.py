@@ -1,0 +1,92 @@
+"""Fixture: the FP64 oracle's DSM error on the 10 x 1024^2, 8-source site.
+
+    python tests/golden/make_c2_oracle_dsm.py [--work DIR] [--out tests/golden/c2_oracle_dsm.json]
+
+Renders the site the benchmark uses for this workload (synth.spec_c2(), 0.5 m
+ground-truth DSM), selects 8 sources per reference view (__auto__, as the
+benchmark), and runs the CPU oracle (oracle/pm_oracle.c, FP64, OpenMP) over
+the whole pipeline the GPU forms run: photometric pass (5 iterations, no
+filter) for all 10 views, geometric pass (5 iterations) from each view's
+photometric result with its sources' photometric depth maps, filter, then
+2.5D fusion with the numpy oracle (oracle/fusion_oracle.py,
+aggregate_2p5d.py:45-89 semantics, <= 2 measurements rejected).  Window
+radius 5 at stride 1 (every tap: the reference's default schedule), 15
+Monte-Carlo samples.  Writes the median |DSM - ground truth| over cells both
+hold, which the GPU tests and the benchmark use as the accuracy bar
+(GPU error <= oracle error + 0.05 m).  Takes ~20 minutes on 8 cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--work", default="/tmp/vsm_c2_oracle")
+    ap.add_argument("--out", default=os.path.join(HERE, "c2_oracle_dsm.json"))
+    ap.add_argument("--iters", type=int, default=5)
+    a = ap.parse_args()
+    from PIL import Image
+
+    import fusion_oracle as fo
+    import pm_oracle as po
+    from vissatsatellitestereo_amd import reparam_depth as rd
+    from vissatsatellitestereo_amd import runtime, synth, workspace
+
+    t_all = time.time()
+    if not os.path.exists(os.path.join(a.work, "colmap", "mvs", "depth_ranges.txt")):
+        synth.build_site(synth.spec_c2(), a.work, device="cpu", save_gt_views=False, dsm_resolution=0.5,
+                         log=lambda *m: print(*m, flush=True))
+        workspace.prepare_dense_workspace(a.work, max_src=8)
+    mvs = os.path.join(a.work, "colmap", "mvs")
+    P4 = rd.projection_matrices(mvs)
+    names = sorted(P4)
+    imgs = {n: np.array(Image.open(os.path.join(mvs, "images", n))) for n in names}
+    dr = rd.read_name_values(os.path.join(mvs, "depth_ranges.txt"))
+    srcs = workspace.select_sources(mvs)
+    params = po.default_params(window_radius=5, window_step=1, num_samples=15)
+    views, photo = {}, {}
+    for k, n in enumerate(names):
+        t = time.time()
+        ov = po.OracleView(imgs[n], [imgs[s] for s in srcs[n]], P4[n], [P4[s] for s in srcs[n]], dr[n],
+                           view_id=k + 1, params=params)
+        photo[n] = ov.solve(0, a.iters)
+        views[n] = ov
+        print(f"photometric {n}: {time.time() - t:.1f}s", flush=True)
+    geo = {}
+    for n in names:
+        t = time.time()
+        ov = views[n]
+        ov.set_source_depths([photo[s]["depth"].reshape(imgs[s].shape).astype(np.float32) for s in srcs[n]])
+        st = {k: v.copy() for k, v in photo[n].items()}
+        geo[n] = ov.solve(1, a.iters, state=st, do_filter=True)["depth"].reshape(imgs[n].shape)
+        print(f"geometric {n}: {time.time() - t:.1f}s, kept {np.mean(geo[n] > 0):.3f}", flush=True)
+    grid = runtime.dsm_grid(runtime.read_aoi(a.work), 0.5, 0.5)
+    stack = np.stack([fo.view_dsm(geo[n].astype(np.float32), np.linalg.inv(P4[n]), grid) for n in names])
+    dsm = fo.fuse(stack, 3)
+    gt = np.load(os.path.join(a.work, "synthetic_gt", "gt_dsm.npy"))
+    ok = ~np.isnan(dsm) & ~np.isnan(gt)
+    err = float(np.median(np.abs(dsm[ok] - gt[ok])))
+    out = dict(workload="10 reference views x 1024x1024, 8 sources (__auto__), photometric + geometric "
+                        f"{a.iters}+{a.iters} iterations, filter, r=5 stride 1, 0.5 m DSM",
+               solver="oracle/pm_oracle.c (FP64) + oracle/fusion_oracle.py",
+               dsm_median_abs_err_m=round(err, 5), dsm_coverage=round(float(ok.mean()), 4),
+               kept_fraction=round(float(np.mean([np.mean(geo[n] > 0) for n in names])), 4),
+               seconds=round(time.time() - t_all, 1), cores=os.cpu_count())
+    print(json.dumps(out), flush=True)
+    with open(a.out, "w") as fp:
+        json.dump(out, fp, indent=1)
+        fp.write("\n")
+
+
+if __name__ == "__main__":
+    main()

@@ -16,14 +16,21 @@ def tiny_spec():
                            azimuths=(0.0, 90.0, 180.0, 270.0), n_boxes=6, n_points=800)
 
 
-def build(name, spec, root, max_src=20, geo=None):
-    """geo=(lat, lon) anchors the site in UTM (aoi.json with a real zone)."""
+def build(name, spec, root, max_src=20, geo=None, save_gt_views=True):
+    """geo=(lat, lon) anchors the site in UTM (aoi.json with a real zone).
+    Sites of more than 4 Mpx in total are rendered on the GPU when there is
+    one (small sites always on the CPU)."""
     key = (name, str(root))
     if key in _CACHE:
         return _CACHE[key]
     wd = os.path.join(str(root), name)
     if not os.path.exists(os.path.join(wd, "colmap", "mvs", "depth_ranges.txt")):
-        synth.build_site(spec, wd, geo=geo)
+        sizes = spec.sizes or [(spec.width, spec.height)] * spec.n_views
+        device = "cpu"
+        if sum(w * h for w, h in sizes) > 4 * 2**20:
+            import torch
+            device = "cuda" if torch.cuda.is_available() else "cpu"
+        synth.build_site(spec, wd, geo=geo, device=device, save_gt_views=save_gt_views)
         workspace.prepare_dense_workspace(wd, max_src=max_src)
     mvs = os.path.join(wd, "colmap", "mvs")
     P4 = rd.projection_matrices(mvs)

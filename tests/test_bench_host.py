@@ -1,0 +1,98 @@
+"""bench.py's step sequence on the CPU with a fake context: a step is one
+half-step of every view, 2 x iterations steps per pass, the photometric
+outputs are kept before the geometric pass begins, and the filter/outputs
+and the fusion run in the solve's last step."""
+import os
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+import bench  # noqa: E402
+
+
+class FakeBatch:
+    def __init__(self, log, views, pass_, graph, nsteps):
+        self.log, self.views, self.pass_, self.graph = log, views, pass_, graph
+        self.left = nsteps
+        log.append(("begin", tuple(v["ref"] for v in views), pass_, graph))
+
+    def step(self, n=1):
+        assert self.left > 0
+        self.left -= n
+        self.log.append(("step", tuple(v["ref"] for v in self.views), self.pass_))
+        return self.left
+
+    def end(self, want=()):
+        assert self.left == 0
+        self.log.append(("end", tuple(v["ref"] for v in self.views), self.pass_))
+        return [{} for _ in self.views]
+
+    def discard(self):
+        self.log.append(("discard", tuple(v["ref"] for v in self.views), self.pass_))
+
+
+class FakeCtx:
+    def __init__(self):
+        self.log, self.params = [], {}
+
+    def set_params(self, **kw):
+        self.params.update(kw)
+
+    def batch(self, views, pass_=0, use_graph=False):
+        assert self.params["geom_consistency"] == pass_ and self.params["filter"] == pass_
+        return FakeBatch(self.log, views, pass_, use_graph, 2 * self.params["num_iterations"])
+
+    def fuse_dsm(self, grid, ids, depth_kind=1, min_measurements=3):
+        self.log.append(("fuse", tuple(ids), depth_kind, min_measurements))
+        return None
+
+
+VIEWS = [dict(ref=r, srcs=[], depth_range=(0, 1)) for r in (1, 2, 3)]
+
+
+def test_batched_steps_make_one_solve_in_20():
+    ctx = FakeCtx()
+    r = bench.FormRunner(ctx, VIEWS, "batched", 2, 5, None)
+    for k in range(20):
+        r.step(k)
+    kinds = [e[0] for e in ctx.log]
+    assert kinds.count("step") == 20 and kinds.count("fuse") == 1
+    begins = [e for e in ctx.log if e[0] == "begin"]
+    assert [b[2] for b in begins] == [0, 1] and all(b[3] for b in begins) and all(len(b[1]) == 3 for b in begins)
+    # the photometric pass ends after its 10th half-step, before the geometric begin
+    i_end0 = ctx.log.index(("end", (1, 2, 3), 0))
+    assert sum(1 for e in ctx.log[:i_end0] if e[0] == "step") == 10
+    assert ctx.log.index(begins[1]) == i_end0 + 1
+    assert ctx.log[-2] == ("end", (1, 2, 3), 1) and ctx.log[-1] == ("fuse", (1, 2, 3), 1, 3)
+    assert r.solves == 1 and len(r.latencies) == 3 and ctx.params["window_step"] == 2
+
+
+def test_plain_steps_run_each_view_alone():
+    ctx = FakeCtx()
+    r = bench.FormRunner(ctx, VIEWS, "plain", 1, 5, None)
+    for k in range(20):
+        r.step(k)
+    steps = [e for e in ctx.log if e[0] == "step"]
+    assert len(steps) == 60 and all(len(e[1]) == 1 for e in steps)
+    # every photometric batch ends before the first geometric one begins
+    last_end0 = max(i for i, e in enumerate(ctx.log) if e[0] == "end" and e[2] == 0)
+    first_begin1 = min(i for i, e in enumerate(ctx.log) if e[0] == "begin" and e[2] == 1)
+    assert last_end0 < first_begin1
+    assert ctx.log[-1] == ("fuse", (1, 2, 3), 1, 3)
+    assert r.solves == 1 and len(r.latencies) == 3 and ctx.params["window_step"] == 1
+
+
+def test_warmup_discards_open_batches_and_photometric_only():
+    ctx = FakeCtx()
+    r = bench.FormRunner(ctx, VIEWS, "batched", 2, 5, None)
+    for k in range(5):
+        r.step(k)
+    r.discard()
+    assert ctx.log[-1][0] == "discard" and r.open == []
+    ctx = FakeCtx()
+    r = bench.FormRunner(ctx, VIEWS[:2], "batched", 2, 5, None, photometric_only=True)
+    for k in range(10):
+        r.step(k)
+    assert r.S == 10 and ctx.log[-1] == ("fuse", (1, 2), 0, 1)
+    assert not any(e[0] == "begin" and e[2] == 1 for e in ctx.log)

@@ -49,7 +49,7 @@ def _gpu_params(p):
             "geom_consistency_regularizer", "geom_consistency_max_cost", "filter_min_ncc",
             "filter_min_triangulation_angle", "filter_geom_consistency_max_cost",
             "filter_min_num_consistent", "perturbation", "normal_perturbation",
-            "max_normal_angle", "perturb_decay", "n_far", "seed")
+            "max_normal_angle", "perturb_decay", "n_far", "seed", "draw_group")
     import pm_oracle as po
     full = po.default_params(**p)
     return {k: full[k] for k in keys if k in full}
@@ -154,6 +154,15 @@ def test_lockstep_photometric_18_sources(many):
     """Half-steps of the kernel for up to 32 sources (no LDS tap-weight table)
     against the oracle."""
     rep, _ = _lockstep(many, _params(window_radius=5, window_step=2), 0, None, 0, 4)
+    for r in rep:
+        assert r["decided_mismatch"] <= 1e-3, r
+        assert r["frac_cost_err"] <= 1e-3, r
+
+
+def test_lockstep_photometric_grouped_draws_18_sources(many):
+    """Source draws shared by 32 x 4-pixel groups (draw_group = 1): the same
+    lockstep agreement with the oracle, which keys its draws the same way."""
+    rep, _ = _lockstep(many, _params(window_radius=5, window_step=2, draw_group=1), 0, None, 0, 4)
     for r in rep:
         assert r["decided_mismatch"] <= 1e-3, r
         assert r["frac_cost_err"] <= 1e-3, r

@@ -301,6 +301,45 @@ def test_draw_counts_follow_weights():
     assert abs(c.var() - 3.75) < 0.3
 
 
+def _group_view(S, W=96, H=16):
+    img = _texture(H, W, 0)
+    return _view(img, [img] * S, [IDENT] * S, params=dict(draw_group=1))
+
+
+def test_grouped_draws_share_uniforms_within_32x4_groups():
+    """draw_group = 1: the 128 pixels of a 32 x 4 group (columns 32i..32i+31,
+    rows 4j..4j+3) draw with the same uniforms -- identical weights give
+    identical counts inside a group, and pixels of different groups draw
+    independently; against its own weights each pixel still follows q."""
+    q = np.array([0.5, 0.0, 0.2, 0.3, 0.0, 1.0])
+    ov = _group_view(len(q))
+    W = ov.W
+    code = po.step_code(0, 3, 0)
+    group = [(y, x) for y in range(4, 8) for x in range(32, 64)]
+    rows = {tuple(ov.draw(y * W + x, code, q)) for y, x in group}
+    assert len(rows) == 1
+    # neighbours across a group boundary (column 31|32, row 3|4) draw apart
+    firsts = [tuple(ov.draw(y * W + x, code, q)) for y, x in ((0, 0), (0, 32), (4, 0), (4, 64), (8, 0))]
+    assert len(set(firsts)) == len(firsts)
+    # the same uniforms against another pixel's weights: the inverse CDF of
+    # those weights (a single positive source takes every draw)
+    assert list(ov.draw(5 * W + 40, code, np.array([0.0, 0.0, 0.0, 0.0, 0.0, 3.0]))) == [0] * 5 + [15]
+    # marginals: one pixel per group over many half-steps follows q
+    tot = np.zeros(len(q))
+    n = 0
+    for it in range(40):
+        for c in (0, 1):
+            for gy in range(4):
+                for gx in range(3):
+                    tot += ov.draw((4 * gy) * W + 32 * gx, po.step_code(0, it, c), q)
+                    n += 1
+    pos = q > 0
+    exp = 15 * n * q[pos] / q.sum()
+    chi2 = float(((tot[pos] - exp) ** 2 / exp).sum())
+    print("grouped draw totals", tot, "expected", exp, "chi2", chi2)
+    assert chi2 < 30.0 and np.all(tot[~pos] == 0)
+
+
 def test_draw_special_cases():
     ov = _draw_view(4)
     code = po.step_code(1, 0, 0)

@@ -422,6 +422,7 @@ PMConst make_const(const vsm_params &p)
     c.n_far = p.n_far;
     c.seed_lo = (uint32_t)p.seed;
     c.seed_hi = (uint32_t)(p.seed >> 32);
+    c.draw_group = p.draw_group;
     return c;
 }
 
@@ -438,6 +439,7 @@ int check_params(vsm_ctx *c, const vsm_params &p)
     if (!(p.sigma_color > 0) || !(p.ncc_sigma > 0) || !(p.incident_angle_sigma > 0))
         return fail(c, VSM_EINVAL, "sigma_color, ncc_sigma and incident_angle_sigma must be > 0");
     if (p.n_far < 1 || p.n_far > 64) return fail(c, VSM_EINVAL, "n_far %d outside [1, 64]", p.n_far);
+    if (p.draw_group != 0 && p.draw_group != 1) return fail(c, VSM_EINVAL, "draw_group must be 0 or 1");
     if (!(p.max_normal_angle > 0 && p.max_normal_angle < 90))
         return fail(c, VSM_EINVAL, "max_normal_angle must be in (0, 90) degrees");
     return VSM_OK;
@@ -880,6 +882,7 @@ void vsm_default_params(vsm_params *p)
     p->perturb_decay = 0.5f;
     p->n_far = 6;
     p->seed = 0x5A7E11173Dull;
+    p->draw_group = 0;
 }
 
 int vsm_create(int device, uint64_t hbm_budget_bytes, vsm_ctx **out)

@@ -769,10 +769,14 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
             }
         } else {
             float tk[kMaxSamples];
+            // the draws' counter word: the pixel, or its 32 x 4 group (one
+            // wave's pixels of a tile) whose pixels share the uniforms
+            const uint32_t dkey = c.draw_group ? (uint32_t)((y >> 2) * ((W + 31) >> 5) + (x >> 5))
+                                               : (uint32_t)p;
 #pragma unroll
             for (int b = 0; b < kMaxSamples / 4; ++b) {
                 float u[4];
-                rng_block(c, V.view_id, (uint32_t)p, code, 2u + (uint32_t)b, u);
+                rng_block(c, V.view_id, dkey, code, 2u + (uint32_t)b, u);
                 tk[4 * b + 0] = u[0] * tot; tk[4 * b + 1] = u[1] * tot;
                 tk[4 * b + 2] = u[2] * tot; tk[4 * b + 3] = u[3] * tot;
             }

@@ -98,10 +98,17 @@ class ServeLoop:
                     self.on_complete(t, view, res)
         return n_active, n_queued
 
+    def _check_progress(self, n_active, n_queued):
+        # the engine has nothing left, yet requests wait: their geometric
+        # pass needs photometric results of views that were never submitted
+        if n_active == 0 and n_queued == 0 and self.pending:
+            raise RuntimeError(f"{len(self.pending)} requests wait for the photometric pass of views "
+                               "that were not submitted")
+
     def drain(self, max_steps=10**7):
         k = 0
         while self.pending and k < max_steps:
-            self.step()
+            self._check_progress(*self.step())
             k += 1
         if self.pending:
             raise RuntimeError(f"{len(self.pending)} requests still pending after {max_steps} steps")
@@ -120,7 +127,9 @@ class ServeLoop:
                 tickets[i] = self.submit(views[i], arrival=start + arrivals[i])
                 nxt += 1
             if self.pending:
-                self.step()
+                na, nq = self.step()
+                if nxt >= len(order):
+                    self._check_progress(na, nq)
                 steps += 1
                 if steps > max_steps:
                     raise RuntimeError("serving did not drain")

@@ -312,6 +312,21 @@ int vsm_fuse_dsm(vsm_ctx *ctx, const vsm_dsm_grid *grid, const int32_t *image_id
 int vsm_proj_to_grid(vsm_ctx *ctx, const double *points, int64_t n, const vsm_dsm_grid *grid,
                      double *dsm_out);
 
+/* proj_to_grid without allocating (no counterpart in the reference): for
+ * callers that cannot allocate device memory -- code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  Same semantics and
+ * bit-identical output as vsm_proj_to_grid, but points_dev (n x 3 FP64) and
+ * dsm_dev (n_size x e_size FP64) are device pointers, the grid may hold at
+ * most VSM_NOALLOC_MAX_CELLS cells (VSM_EINVAL beyond), and the work is
+ * queued on `stream` (a hipStream_t; NULL = the context's stream) without
+ * waiting: the result is ready when that stream is (vsm_synchronize for the
+ * context's stream).  The per-cell scratch is one device array per GPU,
+ * allocated with the library's code and held until the process ends, shared
+ * by every call on that GPU: calls that overlap on one GPU race. */
+#define VSM_NOALLOC_MAX_CELLS (4ll * 1024 * 1024)   /* 32 MiB of 64-bit cell keys */
+int vsm_proj_to_grid_noalloc(vsm_ctx *ctx, const double *points_dev, int64_t n,
+                             const vsm_dsm_grid *grid, double *dsm_dev, void *stream);
+
 /* 3D fusion of depth maps into a point cloud, the path of aggregate_3d.py
  * (aggregate_3d.py:44-82: `colmap stereo_fusion --input_type geometric
  * --StereoFusion.min_num_pixels 4 --StereoFusion.max_reproj_error 2

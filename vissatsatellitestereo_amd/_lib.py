@@ -123,6 +123,8 @@ SIGNATURES = [
     ("vsm_batch_step", _c.c_int, [_c.c_void_p, _c.c_int32, _c.c_int32, P(_c.c_int32)]),
     ("vsm_batch_end", _c.c_int, [_c.c_void_p, _c.c_int32, P(ViewOut), _c.c_int32]),
     ("vsm_set_hbm_budget", _c.c_int, [_c.c_void_p, _c.c_uint64]),
+    ("vsm_proj_to_grid_noalloc", _c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_int64, P(DsmGrid),
+                                            _c.c_void_p, _c.c_void_p]),
     ("vsm_submit", _c.c_int, [_c.c_void_p, P(ViewReq), P(_c.c_uint64)]),
     ("vsm_submit_pass", _c.c_int, [_c.c_void_p, P(ViewReq), _c.c_int32, _c.c_int32, P(_c.c_uint64)]),
     ("vsm_serve_step", _c.c_int, [_c.c_void_p, P(_c.c_int32), P(_c.c_int32)]),

@@ -893,6 +893,8 @@ int vsm_create(int device, uint64_t hbm_budget_bytes, vsm_ctx **out)
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return VSM_EDEVICE;
     if (device < 0 || device >= ndev) return VSM_EINVAL;
     if (hipSetDevice(device) != hipSuccess) return VSM_EDEVICE;
+    // device address of the allocation-free scratch, once per device
+    if (scratch_init(device) != hipSuccess) return VSM_EDEVICE;
     auto *c = new vsm_ctx();
     c->device = device;
     c->budget = hbm_budget_bytes;
@@ -1451,6 +1453,23 @@ int vsm_proj_to_grid(vsm_ctx *c, const double *points, int64_t n, const vsm_dsm_
     HIPCHK(c, hipMemcpyAsync(dsm_out, out, ncell * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     dfree(c, cells); dfree(c, out); dfree(c, pts);
+    return VSM_OK;
+}
+
+int vsm_proj_to_grid_noalloc(vsm_ctx *c, const double *points_dev, int64_t n, const vsm_dsm_grid *grid,
+                             double *dsm_dev, void *stream)
+{
+    if (!c) return VSM_EINVAL;
+    const int rc = check_grid(c, grid);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && !points_dev) || !dsm_dev) return fail(c, VSM_EINVAL, "bad points/output");
+    const long long ncell = (long long)grid->e_size * grid->n_size;
+    if (ncell > kNoAllocMaxCells)
+        return fail(c, VSM_EINVAL, "grid of %lld cells exceeds the %lld cells of the allocation-free "
+                    "scratch", ncell, (long long)kNoAllocMaxCells);
+    HIPCHK(c, hipSetDevice(c->device));
+    const hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    HIPCHK(c, proj_to_grid_noalloc(c->device, points_dev, n, grid_dev(*grid), dsm_dev, st));
     return VSM_OK;
 }
 

@@ -23,6 +23,9 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include <mutex>
+#include <vector>
+
 namespace vsm {
 
 __device__ __forceinline__ unsigned long long enc_f64(double v)
@@ -469,6 +472,51 @@ hipError_t launch_fill_u64n(unsigned long long *p, long long n, unsigned long lo
     if (n <= 0) return hipSuccess;
     k_fill_u64n<<<nblk(n, 256), 256, 0, st>>>(p, n, v);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free rasteriser.  Its only scratch, the per-cell 64-bit max keys,
+// is this file-scope device array (each device holds its own copy for the
+// life of the process).  Host code never passes the symbol itself to a kernel:
+// it uses the device address taken once per device by scratch_init.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned long long scratch[kNoAllocMaxCells];
+
+static std::mutex g_scratch_mu;
+static std::vector<unsigned long long *> g_scratch_ptr;   // per device ordinal
+
+hipError_t scratch_init(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device) return hipErrorInvalidDevice;      // the caller makes `device` current
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = static_cast<unsigned long long *>(p);
+    return hipSuccess;
+}
+
+hipError_t proj_to_grid_noalloc(int device, const double *pts, long long n, DsmGridDev g,
+                                double *dsm, hipStream_t st)
+{
+    unsigned long long *cells = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (device >= 0 && device < (int)g_scratch_ptr.size()) cells = g_scratch_ptr[device];
+    }
+    if (!cells) return hipErrorNotInitialized;
+    const long long ncell = (long long)g.e_size * g.n_size;
+    if (ncell <= 0 || ncell > kNoAllocMaxCells) return hipErrorInvalidValue;
+    // the previous call's keys are still there: clear this call's cells
+    hipError_t e = launch_fill_u64n(cells, ncell, 0ull, st);
+    if (e == hipSuccess) e = launch_rasterize_points(pts, n, g, cells, st);
+    if (e == hipSuccess) e = launch_decode_fill_f64(cells, g.e_size, g.n_size, dsm, st);
+    return e;
 }
 
 }  // namespace vsm

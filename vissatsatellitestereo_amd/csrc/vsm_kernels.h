@@ -78,4 +78,13 @@ hipError_t launch_fill_u32(unsigned int *p, long long n, unsigned int v, hipStre
 hipError_t launch_fill_u64n(unsigned long long *p, long long n, unsigned long long v,
                             hipStream_t st);
 
+// Allocation-free proj_to_grid (vsm_fuse.hip): the cell keys live in a
+// file-scope device array of kNoAllocMaxCells entries.  scratch_init takes
+// its address on `device` (which must be current), once per device;
+// proj_to_grid_noalloc then rasterises on stream st without allocating.
+constexpr long long kNoAllocMaxCells = VSM_NOALLOC_MAX_CELLS;
+hipError_t scratch_init(int device);
+hipError_t proj_to_grid_noalloc(int device, const double *pts, long long n, DsmGridDev g,
+                                double *dsm, hipStream_t st);
+
 }  // namespace vsm

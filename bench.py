@@ -385,7 +385,8 @@ def cpu_baseline_fusion(ctx, site, grid, max_views=2, band_rows=256):
 def make_ctx(sites, a):
     from vissatsatellitestereo_amd.mvs import MvsContext
     ctx = MvsContext(device=0, hbm_budget_bytes=int(a.hbm_budget_gb * 1e9),
-                     window_radius=a.window_radius, window_step=1, num_samples=15)
+                     window_radius=a.window_radius, window_step=1, num_samples=15,
+                     draw_group=a.draw_group)
     for s in sites:
         for n in s["names"]:
             ctx.add_image(s["ids"][n], s["imgs"][n], s["P4"][n])
@@ -516,7 +517,8 @@ def bench_forms(a, cfg, steps, warmup, with_cpu=True):
                    "forms": {"plain": "one view per call, window step 1",
                              "batched": f"all views per launch, persistent kernel, hipGraph replay, "
                                         f"window step {step} (autotuner)"},
-                   "autotune": tune_rep},
+                   "autotune": tune_rep,
+                   "source_draws": "per 32x4-pixel group" if a.draw_group else "per pixel"},
         "plain_views_per_s": round(p["views_per_s"], 4),
         "batched_over_plain": round(b["views_per_s"] / p["views_per_s"], 3),
         "batched_over_plain_same_step": same,
@@ -732,6 +734,8 @@ def main():
                     help="c5: steps of the plain form at the batched window step")
     ap.add_argument("--no-nested", action="store_true", help="c5: skip the nested c2 workload")
     ap.add_argument("--cpu-crop", type=int, default=512)
+    ap.add_argument("--draw-group", type=int, default=0, choices=(0, 1),
+                    help="Monte-Carlo source draws keyed by pixel (0) or by 32x4-pixel group (1)")
     ap.add_argument("--forms", default="plain,batched", help="comma list of plain,batched")
     ap.add_argument("--build-only", action="store_true", help="build the synthetic site(s) and exit")
     ap.add_argument("--copy-calibration-mb", type=int, default=0)

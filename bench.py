@@ -144,9 +144,11 @@ class FormRunner:
     """The step sequence of one form over a set of views (see the module
     docstring): step k runs half-step k mod S of solve k // S for every view."""
 
-    def __init__(self, ctx, views, form, window_step, iters, grid, photometric_only=False):
+    def __init__(self, ctx, views, form, window_step, iters, grid, photometric_only=False,
+                 tile_width=32):
         self.ctx, self.views, self.form = ctx, list(views), form
         self.window_step, self.iters, self.grid = window_step, iters, grid
+        self.tile_width = tile_width
         self.photometric_only = photometric_only
         self.P = 2 * iters
         self.S = self.P if photometric_only else 2 * self.P
@@ -158,7 +160,8 @@ class FormRunner:
         self.solves = 0
 
     def _params(self, pass_):
-        self.ctx.set_params(num_iterations=self.iters, window_step=self.window_step, **PASS_PARAMS[pass_])
+        self.ctx.set_params(num_iterations=self.iters, window_step=self.window_step,
+                            tile_width=self.tile_width, **PASS_PARAMS[pass_])
 
     def _fuse(self):
         ids = [v["ref"] for v in self.views]
@@ -225,15 +228,15 @@ class FormRunner:
 
 
 def run_form(ctx, views, form, window_step, a, grid, steps, warmup, photometric_only=False,
-             timing=True):
+             timing=True, tile_width=32):
     """Warmup steps on a discarded solve, then `steps` timed steps from a
     fresh solve.  Returns per-step walls, the runner and the device stats of
     the timed steps."""
-    r = FormRunner(ctx, views, form, window_step, a.iters, grid, photometric_only)
+    r = FormRunner(ctx, views, form, window_step, a.iters, grid, photometric_only, tile_width)
     for k in range(warmup):
         r.step(k)
     r.discard()
-    r = FormRunner(ctx, views, form, window_step, a.iters, grid, photometric_only)
+    r = FormRunner(ctx, views, form, window_step, a.iters, grid, photometric_only, tile_width)
     ctx.reset_stats()
     ctx.set_timing(timing)
     walls = []
@@ -395,12 +398,18 @@ def make_ctx(sites, a):
 
 
 def choose_step(ctx, views, a):
+    """Window step and tile width of the batched form: the autotuner's, or
+    the command line's."""
     if a.batched_step != "auto":
-        return int(a.batched_step), {"window_step": int(a.batched_step), "source": "command line"}
+        tw = 32 if a.tile_width == "auto" else int(a.tile_width)
+        return int(a.batched_step), {"window_step": int(a.batched_step), "tile_width": tw,
+                                     "source": "command line"}
     from vissatsatellitestereo_amd import autotune
     steps = tuple(int(x) for x in a.tune_steps.split(","))
+    widths = autotune.TILE_WIDTHS if a.tile_width == "auto" else (int(a.tile_width),)
     st, rep = autotune.tune(ctx, views, iters=a.iters, radius=a.window_radius, steps=steps,
-                            use_cache=not a.no_tune_cache)
+                            use_cache=not a.no_tune_cache, widths=widths)
+    rep.setdefault("tile_width", widths[0])
     log("autotune", rep)
     return st, rep
 
@@ -449,9 +458,11 @@ def bench_forms(a, cfg, steps, warmup, with_cpu=True):
     views = site["views"]
     step, tune_rep = choose_step(ctx, views, a)
     res = {}
+    tile_w = tune_rep["tile_width"]
     for form in [f for f in ("batched", "plain") if f in a.forms.split(",")]:
         ws = 1 if form == "plain" else step
-        walls, r, st = run_form(ctx, views, form, ws, a, grid, steps, warmup, photometric_only)
+        walls, r, st = run_form(ctx, views, form, ws, a, grid, steps, warmup, photometric_only,
+                                tile_width=32 if form == "plain" else tile_w)
         res[form] = summarize(walls, r, len(views), gt)
         res[form]["stats"] = st
         log(form, {k: v for k, v in res[form].items() if k not in ("stats", "step_ms")})
@@ -464,7 +475,7 @@ def bench_forms(a, cfg, steps, warmup, with_cpu=True):
     if "plain" in res and "batched" in res and step != 1 and not a.no_same_step:
         m = steps if cfg != "c5" else min(steps, a.same_step_steps)
         walls, r, _ = run_form(ctx, views, "plain", step, a, grid, m, min(warmup, 1), photometric_only,
-                               timing=False)
+                               timing=False, tile_width=tile_w)
         t_plain = float(np.sum(walls))
         t_batched = float(np.sum(b["step_ms"][:m])) / 1e3
         same = {"window_step": step, "steps": m,
@@ -488,7 +499,8 @@ def bench_forms(a, cfg, steps, warmup, with_cpu=True):
                     ctx.get_depth_map(v["ref"], kind=1)
             except VsmError:
                 for pass_ in (0, 1):
-                    ctx.set_params(num_iterations=a.iters, window_step=step, **PASS_PARAMS[pass_])
+                    ctx.set_params(num_iterations=a.iters, window_step=step, tile_width=tile_w,
+                                   **PASS_PARAMS[pass_])
                     ctx.solve(views, pass_, want=(), use_graph=True)
             log("cpu baseline: PatchMatch oracle on a crop")
             cpu = cpu_baseline(ctx, site, a.window_radius, a.iters, crop=a.cpu_crop)
@@ -514,9 +526,10 @@ def bench_forms(a, cfg, steps, warmup, with_cpu=True):
         "config": {"workload": f"{cfg}: {desc}, photometric+geometric {a.iters}+{a.iters} iterations, "
                                f"filter, r={a.window_radius}, {res_m} m DSM fusion; a step is one "
                                f"half-step of every view, {S} steps one solve",
-                   "forms": {"plain": "one view per call, window step 1",
+                   "forms": {"plain": "one view per call, window step 1, 32 x 16 tiles",
                              "batched": f"all views per launch, persistent kernel, hipGraph replay, "
-                                        f"window step {step} (autotuner)"},
+                                        f"window step {step}, {tile_w} x {512 // tile_w} tiles "
+                                        f"(autotuner)"},
                    "autotune": tune_rep,
                    "source_draws": "per 32x4-pixel group" if a.draw_group else "per pixel"},
         "plain_views_per_s": round(p["views_per_s"], 4),
@@ -600,7 +613,8 @@ def bench_serving(a):
         errs = [e for e in errs if e is not None]
         return round(float(np.median(errs)), 4) if errs else None
 
-    ctx.set_params(num_iterations=a.iters, window_step=step, **PASS_PARAMS[1])
+    tile_w = tune_rep["tile_width"]
+    ctx.set_params(num_iterations=a.iters, window_step=step, tile_width=tile_w, **PASS_PARAMS[1])
     warm = ServeLoop(ctx)                              # two photometric requests end to end
     for v in sites[0]["views"][:2]:
         warm.submit(v)
@@ -615,7 +629,7 @@ def bench_serving(a):
     # plain: one solve at a time in arrival order, each a single view at
     # window step 1; a view's geometric solve runs once the photometric
     # solves of the view and its sources are done
-    ctx.set_params(window_step=1)
+    ctx.set_params(window_step=1, tile_width=32)
     plain = queue_schedule(ctx, arrivals, reqs)
     for k in list(dsm):
         dsm[k] = ctx.fuse_dsm(grids[k], finished[k], depth_kind=1, min_measurements=3)
@@ -623,9 +637,9 @@ def bench_serving(a):
     log(f"queue: {plain['views_per_s']:.2f} views/s", "dsm", queue_err)
     # capacity: every request queued at t = 0 (at the configured rate the
     # served form keeps up with the arrivals and its views/s is the offered load)
-    ctx.set_params(window_step=step)
+    ctx.set_params(window_step=step, tile_width=tile_w)
     burst = ServeLoop(ctx, geometric=True).run_schedule([0.0] * len(reqs), reqs)
-    ctx.set_params(window_step=1)
+    ctx.set_params(window_step=1, tile_width=32)
     plain_cap = queue_schedule(ctx, [0.0] * len(reqs), reqs)
     log(f"capacity: served {burst['views_per_s']:.2f} views/s, plain {plain_cap['views_per_s']:.2f} views/s")
     out = {
@@ -728,14 +742,16 @@ def main():
                     help="window-step candidates of the autotuner (the batched configuration "
                          "allows strides 1 and 2; wider ones only on request)")
     ap.add_argument("--no-tune-cache", action="store_true")
+    ap.add_argument("--tile-width", default="auto", help="auto (autotuner) or 16, 32, 64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-same-step", action="store_true")
     ap.add_argument("--same-step-steps", type=int, default=4,
                     help="c5: steps of the plain form at the batched window step")
     ap.add_argument("--no-nested", action="store_true", help="c5: skip the nested c2 workload")
     ap.add_argument("--cpu-crop", type=int, default=512)
-    ap.add_argument("--draw-group", type=int, default=0, choices=(0, 1),
-                    help="Monte-Carlo source draws keyed by pixel (0) or by 32x4-pixel group (1)")
+    ap.add_argument("--draw-group", type=int, default=1, choices=(0, 1),
+                    help="Monte-Carlo source draws keyed by pixel (0) or by 32x4-pixel group (1, "
+                         "both forms)")
     ap.add_argument("--forms", default="plain,batched", help="comma list of plain,batched")
     ap.add_argument("--build-only", action="store_true", help="build the synthetic site(s) and exit")
     ap.add_argument("--copy-calibration-mb", type=int, default=0)

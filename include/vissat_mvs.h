@@ -72,11 +72,19 @@ typedef struct {
     int32_t n_far;                    /* propagation candidates per direction */
     uint64_t seed;
     /* Counter word of the Monte-Carlo source draws: 0 = the pixel (every
-     * pixel draws independently), 1 = its group of 32 x 4 pixels (columns
-     * 32i..32i+31, rows 4j..4j+3: each pixel still draws from its own
+     * pixel draws independently), 1 = its group of tile_width x
+     * (128 / tile_width) pixels, the pixels one wave of the half-step kernel
+     * covers (32 x 4 for 32-wide tiles): each pixel still draws from its own
      * weights, with the uniforms its group shares, so neighbouring pixels
-     * with similar weights draw the same sources). */
+     * with similar weights draw the same sources and a wave runs fewer
+     * window passes. */
     int32_t draw_group;
+    /* Half-step tile width in pixels, 16, 32 or 64 (tiles of 16 x 32,
+     * 32 x 16 or 64 x 8; 256 threads, one pixel of the active colour each):
+     * a schedule knob of the batched form chosen by the autotuner.  Results
+     * agree across widths to FP32 rounding (the per-tile FP64 re-centring of
+     * the source maps sits at the tile centre). */
+    int32_t tile_width;
 } vsm_params;
 
 /* One reference view to solve. */

@@ -69,7 +69,9 @@ typedef struct {
     double perturb_decay;          /* per iteration */
     int32_t n_far;                 /* candidates per direction */
     uint64_t seed;
-    int32_t draw_group;            /* 1: source draws keyed by 32x4-pixel group, 0: by pixel */
+    int32_t draw_group;            /* 0: source draws keyed by pixel; else the width (16, 32, 64)
+                                      of the pixel groups, 128 / width rows tall, whose pixels
+                                      share the draws' uniforms */
 } orc_params;
 
 typedef struct {
@@ -341,15 +343,15 @@ void orc_reset_state(const orc_view *v, double *cost, double *prob)
 }
 
 /* Counter word of a pixel's Monte-Carlo draws: the pixel index, or with
- * draw_group the index of its group of 32 x 4 pixels (columns 32i..32i+31,
- * rows 4j..4j+3), so every pixel of a group uses the same uniforms (each
- * against its own weights). */
+ * draw_group = g the index of its group of g x (128 / g) pixels (columns
+ * g i .. g i + g - 1, rows (128/g) j .. (128/g) j + 128/g - 1), so every
+ * pixel of a group uses the same uniforms (each against its own weights). */
 static inline uint32_t draw_key(const orc_view *v, const orc_params *p, uint32_t pixel)
 {
     if (!p->draw_group) return pixel;
-    const uint32_t W = (uint32_t)v->width;
+    const uint32_t W = (uint32_t)v->width, gw = (uint32_t)p->draw_group, gh = 128u / gw;
     const uint32_t x = pixel % W, y = pixel / W;
-    return (y >> 2) * ((W + 31u) >> 5) + (x >> 5);
+    return (y / gh) * ((W + gw - 1u) / gw) + x / gw;
 }
 
 /* Monte-Carlo view selection: num_samples draws of a source with probability

@@ -101,6 +101,13 @@ def default_params(**kw):
     return p
 
 
+def _draw_group_width(p):
+    """The oracle's draw key: 0 (per pixel) or the width of the pixel groups
+    sharing the draws' uniforms -- the GPU's draw_group = 1 groups the pixels
+    of one wave of its half-step kernel, tile_width x (128 / tile_width)."""
+    return int(p.get("tile_width", 32)) if p.get("draw_group", 0) else 0
+
+
 def to_orc_params(p):
     ss = p["sigma_spatial"] if p["sigma_spatial"] > 0 else float(p["window_radius"])
     return OrcParams(p["window_radius"], p["window_step"], ss, p["sigma_color"], p["ncc_sigma"],
@@ -111,7 +118,7 @@ def to_orc_params(p):
                      p["filter_geom_consistency_max_cost"], p["filter_min_num_consistent"],
                      p["depth_min"], p["depth_max"], p["perturbation"], p["normal_perturbation"],
                      math.radians(p["max_normal_angle"]), p["perturb_decay"], p["n_far"],
-                     p["seed"], int(p.get("draw_group", 0)))
+                     p["seed"], _draw_group_width(p))
 
 
 def camera_center(P4):

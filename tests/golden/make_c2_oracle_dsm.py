@@ -53,7 +53,8 @@ def main():
     imgs = {n: np.array(Image.open(os.path.join(mvs, "images", n))) for n in names}
     dr = rd.read_name_values(os.path.join(mvs, "depth_ranges.txt"))
     srcs = workspace.select_sources(mvs)
-    params = po.default_params(window_radius=5, window_step=1, num_samples=15)
+    # the solver parameters; each view brings its own depth range
+    params = dict(window_radius=5, window_step=1, num_samples=15, draw_group=0)
     views, photo = {}, {}
     for k, n in enumerate(names):
         t = time.time()

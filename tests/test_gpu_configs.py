@@ -46,14 +46,14 @@ GPU_KEYS = ("window_radius", "window_step", "num_samples", "num_iterations", "si
             "geom_consistency_regularizer", "geom_consistency_max_cost", "filter_min_ncc",
             "filter_min_triangulation_angle", "filter_geom_consistency_max_cost",
             "filter_min_num_consistent", "perturbation", "normal_perturbation", "max_normal_angle",
-            "perturb_decay", "n_far", "seed", "draw_group")
+            "perturb_decay", "n_far", "seed", "draw_group", "tile_width")
 
 
 def _ctx(site, p, ids=None, budget=200 * 10**9):
     import pm_oracle as po
     from vissatsatellitestereo_amd.mvs import MvsContext
     full = po.default_params(**p)
-    ctx = MvsContext(device=0, hbm_budget_bytes=budget, **{k: full[k] for k in GPU_KEYS})
+    ctx = MvsContext(device=0, hbm_budget_bytes=budget, **{k: full[k] for k in GPU_KEYS if k in full})
     for n in (ids or site["names"]):
         ctx.add_image(_id(site, n), site["imgs"][n], site["P4"][n])
     return ctx

@@ -49,7 +49,7 @@ def _gpu_params(p):
             "geom_consistency_regularizer", "geom_consistency_max_cost", "filter_min_ncc",
             "filter_min_triangulation_angle", "filter_geom_consistency_max_cost",
             "filter_min_num_consistent", "perturbation", "normal_perturbation",
-            "max_normal_angle", "perturb_decay", "n_far", "seed", "draw_group")
+            "max_normal_angle", "perturb_decay", "n_far", "seed", "draw_group", "tile_width")
     import pm_oracle as po
     full = po.default_params(**p)
     return {k: full[k] for k in keys if k in full}
@@ -159,10 +159,13 @@ def test_lockstep_photometric_18_sources(many):
         assert r["frac_cost_err"] <= 1e-3, r
 
 
-def test_lockstep_photometric_grouped_draws_18_sources(many):
+@pytest.mark.parametrize("tile_width", [16, 32])
+def test_lockstep_photometric_grouped_draws_18_sources(many, tile_width):
     """Source draws shared by 32 x 4-pixel groups (draw_group = 1): the same
-    lockstep agreement with the oracle, which keys its draws the same way."""
-    rep, _ = _lockstep(many, _params(window_radius=5, window_step=2, draw_group=1), 0, None, 0, 4)
+    lockstep agreement with the oracle, which keys its draws the same way, in
+    the kernel for more than 16 sources at both of its tile shapes."""
+    rep, _ = _lockstep(many, _params(window_radius=5, window_step=2, draw_group=1, tile_width=tile_width),
+                       0, None, 0, 4)
     for r in rep:
         assert r["decided_mismatch"] <= 1e-3, r
         assert r["frac_cost_err"] <= 1e-3, r
@@ -255,10 +258,12 @@ def test_lockstep_photometric_multisource(tiny):
         assert r["frac_cost_err"] <= 1e-3, r
 
 
-def test_lockstep_photometric_strided(tiny):
+@pytest.mark.parametrize("tile_width", [16, 32, 64])
+def test_lockstep_photometric_strided(tiny, tile_width):
     """r = 5 at window step 2: the 36-tap window of the batched form (tap
-    weights kept in LDS) against the oracle, half-step by half-step."""
-    rep, _ = _lockstep(tiny, _params(window_radius=5, window_step=2), 0, 3, 0, 6)
+    weights kept in LDS) against the oracle, half-step by half-step, for every
+    tile shape of the half-step kernel (16 x 32, 32 x 16, 64 x 8)."""
+    rep, _ = _lockstep(tiny, _params(window_radius=5, window_step=2, tile_width=tile_width), 0, 3, 0, 6)
     for r in rep:
         assert r["decided_mismatch"] <= 1e-3, r
         assert r["frac_cost_err"] <= 1e-3, r
@@ -290,8 +295,9 @@ def test_full_solve_accuracy_and_oracle_agreement(c1):
     assert np.all(nz[:, 2] <= 0)
 
 
-def test_batched_equals_single(tiny):
-    p = _params(window_radius=4, num_iterations=3)
+@pytest.mark.parametrize("tile_width", [16, 32, 64])
+def test_batched_equals_single(tiny, tile_width):
+    p = _params(window_radius=4, num_iterations=3, tile_width=tile_width)
     ctx, _, _ = _setup(tiny, p)
     views = []
     for r in tiny["names"]:

@@ -231,6 +231,7 @@ def test_autotune_picks_and_caches(tiny, tmp_path):
     st, rep = autotune.tune(ctx, views, iters=3, radius=4, cache_path=cache)
     print("autotune", rep)
     assert st in (1, 2) and not rep["cached"] and rep["calib_views"] == 4
+    assert rep["tile_width"] in (16, 32, 64) and set(rep["tile_seconds"]) == {"16", "32", "64"}
     assert rep["kept_fraction"][str(st)] >= rep["kept_fraction"]["1"] - rep["max_drop"]
     st2, rep2 = autotune.tune(ctx, views, iters=3, radius=4, cache_path=cache)
     assert st2 == st and rep2["cached"]

@@ -301,37 +301,41 @@ def test_draw_counts_follow_weights():
     assert abs(c.var() - 3.75) < 0.3
 
 
-def _group_view(S, W=96, H=16):
+def _group_view(S, gw, W=192, H=32):
     img = _texture(H, W, 0)
-    return _view(img, [img] * S, [IDENT] * S, params=dict(draw_group=1))
+    return _view(img, [img] * S, [IDENT] * S, params=dict(draw_group=1, tile_width=gw))
 
 
-def test_grouped_draws_share_uniforms_within_32x4_groups():
-    """draw_group = 1: the 128 pixels of a 32 x 4 group (columns 32i..32i+31,
-    rows 4j..4j+3) draw with the same uniforms -- identical weights give
-    identical counts inside a group, and pixels of different groups draw
-    independently; against its own weights each pixel still follows q."""
+@pytest.mark.parametrize("gw", [16, 32, 64])
+def test_grouped_draws_share_uniforms_within_wave_groups(gw):
+    """draw_group = 1 with tiles gw wide: the 128 pixels of a gw x (128/gw)
+    group (the pixels one wave of the half-step kernel covers) draw with the
+    same uniforms -- identical weights give identical counts inside a group,
+    and pixels of different groups draw independently; against its own
+    weights each pixel still follows q."""
+    gh = 128 // gw
     q = np.array([0.5, 0.0, 0.2, 0.3, 0.0, 1.0])
-    ov = _group_view(len(q))
+    ov = _group_view(len(q), gw)
     W = ov.W
     code = po.step_code(0, 3, 0)
-    group = [(y, x) for y in range(4, 8) for x in range(32, 64)]
+    group = [(y, x) for y in range(gh, 2 * gh) for x in range(gw, 2 * gw)]
     rows = {tuple(ov.draw(y * W + x, code, q)) for y, x in group}
     assert len(rows) == 1
-    # neighbours across a group boundary (column 31|32, row 3|4) draw apart
-    firsts = [tuple(ov.draw(y * W + x, code, q)) for y, x in ((0, 0), (0, 32), (4, 0), (4, 64), (8, 0))]
-    assert len(set(firsts)) == len(firsts)
+    # neighbours across a group boundary (column gw-1|gw, row gh-1|gh) draw apart
+    firsts = [tuple(ov.draw(y * W + x, code, q))
+              for y, x in ((0, 0), (0, gw), (gh, 0), (gh, 2 * gw), (2 * gh, 0), (gh - 1, gw - 1))]
+    assert len(set(firsts[:5])) == 5 and firsts[5] == firsts[0]
     # the same uniforms against another pixel's weights: the inverse CDF of
     # those weights (a single positive source takes every draw)
-    assert list(ov.draw(5 * W + 40, code, np.array([0.0, 0.0, 0.0, 0.0, 0.0, 3.0]))) == [0] * 5 + [15]
+    assert list(ov.draw((gh + 1) * W + gw + 3, code, np.array([0.0, 0.0, 0.0, 0.0, 0.0, 3.0]))) == [0] * 5 + [15]
     # marginals: one pixel per group over many half-steps follows q
     tot = np.zeros(len(q))
     n = 0
     for it in range(40):
         for c in (0, 1):
-            for gy in range(4):
+            for gy in range(32 // gh):
                 for gx in range(3):
-                    tot += ov.draw((4 * gy) * W + 32 * gx, po.step_code(0, it, c), q)
+                    tot += ov.draw((gh * gy) * W + gw * gx, po.step_code(0, it, c), q)
                     n += 1
     pos = q > 0
     exp = 15 * n * q[pos] / q.sum()

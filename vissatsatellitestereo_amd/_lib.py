@@ -40,7 +40,8 @@ class Params(ctypes.Structure):
                 ("depth_max", ctypes.c_float), ("perturbation", ctypes.c_float),
                 ("normal_perturbation", ctypes.c_float), ("max_normal_angle", ctypes.c_float),
                 ("perturb_decay", ctypes.c_float), ("n_far", ctypes.c_int32),
-                ("seed", ctypes.c_uint64), ("draw_group", ctypes.c_int32)]
+                ("seed", ctypes.c_uint64), ("draw_group", ctypes.c_int32),
+                ("tile_width", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

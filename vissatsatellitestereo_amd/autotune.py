@@ -14,6 +14,17 @@ forward-backward reprojection.  A candidate is accepted when its kept
 fraction is within ``max_drop`` of stride 1's; the fastest accepted one (wall
 time of both passes) wins.  Results are cached in a JSON file keyed by
 (height, width, sources, radius, iterations) size classes and the candidate set.
+
+Second knob: the half-step kernel's tile shape (``tile_width`` 16, 32 or 64:
+tiles of 16 x 32, 32 x 16 or 64 x 8 pixels; a wave covers 8, 4 or 2 tile
+rows, which changes the footprint its gathers touch in each source image).
+The shape is a schedule only (results agree to FP32 rounding), so it is
+chosen by time alone: with the chosen window step, a few views of the batch
+with their full source lists run eight photometric half-steps per candidate,
+the last four timed (past the first iterations' random hypotheses); the
+fastest wins.  64-wide tiles are offered only for views with at most 16
+sources.  With grouped source draws (draw_group = 1) the draw groups follow
+the tile shape, so the candidates are timed with their own grouping.
 """
 import json
 import os
@@ -30,9 +41,42 @@ def size_class(n):
     return 1 << int(np.ceil(np.log2(max(1, n))))
 
 
-def key_of(shape, n_src, radius, iters, steps=(1, 2)):
+TILE_WIDTHS = (16, 32, 64)
+
+
+def key_of(shape, n_src, radius, iters, steps=(1, 2), widths=TILE_WIDTHS):
     cand = "".join(str(int(s)) for s in steps)
-    return f"h{size_class(shape[0])}_w{size_class(shape[1])}_s{n_src}_r{radius}_i{iters}_c{cand}"
+    tw = "-".join(str(int(w)) for w in widths)
+    return f"h{size_class(shape[0])}_w{size_class(shape[1])}_s{n_src}_r{radius}_i{iters}_c{cand}_t{tw}"
+
+
+def tune_tiles(ctx, views, step, widths=TILE_WIDTHS, n_views=4, halfsteps=4):
+    """Seconds per candidate tile width: photometric half-steps
+    halfsteps .. 2 halfsteps - 1 of the first n_views views (full source
+    lists), after the first `halfsteps` untimed."""
+    calib = views[:n_views]
+    n_src = max(len(v["srcs"]) for v in calib)
+    saved = ctx.params
+    times = {}
+    try:
+        for tw in widths:
+            if tw == 64 and n_src > 16:
+                continue
+            ctx.set_params(window_step=step, tile_width=tw, geom_consistency=0, filter=0,
+                           num_iterations=max(int(saved.num_iterations), halfsteps))
+            b = ctx.batch(calib, 0)
+            b.step(halfsteps)                                     # random start, warm caches
+            ctx.synchronize()
+            t = time.perf_counter()
+            b.step(halfsteps)
+            ctx.synchronize()
+            times[tw] = time.perf_counter() - t
+            b.discard()
+    finally:
+        ctx.set_params(window_step=saved.window_step, tile_width=saved.tile_width,
+                       geom_consistency=saved.geom_consistency, filter=saved.filter,
+                       num_iterations=saved.num_iterations)
+    return times
 
 
 def load_cache(path=CACHE):
@@ -52,8 +96,9 @@ def save_cache(cache, path=CACHE):
 
 
 def tune(ctx, views, iters, radius, steps=(1, 2), max_drop=0.02, calib_views=8,
-         cache_path=CACHE, use_cache=True):
-    """Pick window_step for `views` on `ctx`; returns (step, report).
+         cache_path=CACHE, use_cache=True, widths=TILE_WIDTHS):
+    """Pick window_step and tile_width for `views` on `ctx`; returns
+    (step, report) with report["tile_width"] the chosen width.
 
     Note: the calibration solves replace the context's stored photometric
     and geometric maps of the calibration views."""
@@ -62,7 +107,8 @@ def tune(ctx, views, iters, radius, steps=(1, 2), max_drop=0.02, calib_views=8,
     steps = tuple(int(s) for s in steps if 1 <= int(s) <= radius)
     if not steps or steps[0] != 1:
         raise ValueError("window-step candidates must start with 1 (the accuracy reference)")
-    key = key_of(shape, n_src, radius, iters, steps)
+    widths = tuple(int(w) for w in widths)
+    key = key_of(shape, n_src, radius, iters, steps, widths)
     cache = load_cache(cache_path) if use_cache else {}
     if key in cache:
         return cache[key]["window_step"], dict(cache[key], cached=True)
@@ -93,8 +139,11 @@ def tune(ctx, views, iters, radius, steps=(1, 2), max_drop=0.02, calib_views=8,
                        geom_consistency=saved.geom_consistency, filter=saved.filter)
     ok = [st for st in steps if kept[st] >= kept[steps[0]] - max_drop]
     best = min(ok, key=lambda st: times[st])
-    rep = dict(window_step=best, key=key, seconds={str(k): v for k, v in times.items()},
+    tile_times = tune_tiles(ctx, views, best, widths) if len(widths) > 1 else {}
+    tile = min(tile_times, key=lambda w: tile_times[w]) if tile_times else int(ctx.params.tile_width)
+    rep = dict(window_step=best, tile_width=tile, key=key, seconds={str(k): v for k, v in times.items()},
                kept_fraction={str(k): v for k, v in kept.items()}, max_drop=max_drop,
+               tile_seconds={str(k): v for k, v in tile_times.items()},
                calib_views=len(calib), cached=False)
     if use_cache:
         cache[key] = rep

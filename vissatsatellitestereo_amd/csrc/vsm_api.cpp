@@ -101,7 +101,7 @@ struct vsm_ctx {
     unsigned int *d_work = nullptr;        // per-XCD tile cursors of the persistent kernel
     float *d_c0 = nullptr;                 // per-workgroup view-selection costs (> 16 sources)
     size_t c0_floats = 0;
-    int grid[3] = {0, 0, 0};               // resident workgroups per MAXS variant
+    int grid[9] = {0};                     // resident workgroups per (MAXS, tile width) variant
     bool timing = false;
     double halfstep_ms = 0.0;
     int64_t halfstep_launches = 0;
@@ -422,7 +422,7 @@ PMConst make_const(const vsm_params &p)
     c.n_far = p.n_far;
     c.seed_lo = (uint32_t)p.seed;
     c.seed_hi = (uint32_t)(p.seed >> 32);
-    c.draw_group = p.draw_group;
+    c.draw_group = p.draw_group ? p.tile_width : 0;     // group width in pixels, or per pixel
     return c;
 }
 
@@ -440,6 +440,8 @@ int check_params(vsm_ctx *c, const vsm_params &p)
         return fail(c, VSM_EINVAL, "sigma_color, ncc_sigma and incident_angle_sigma must be > 0");
     if (p.n_far < 1 || p.n_far > 64) return fail(c, VSM_EINVAL, "n_far %d outside [1, 64]", p.n_far);
     if (p.draw_group != 0 && p.draw_group != 1) return fail(c, VSM_EINVAL, "draw_group must be 0 or 1");
+    if (!tile_width_supported(p.tile_width, 1))
+        return fail(c, VSM_EINVAL, "tile_width %d: the half-step kernel has tiles 16, 32 or 64 wide", p.tile_width);
     if (!(p.max_normal_angle > 0 && p.max_normal_angle < 90))
         return fail(c, VSM_EINVAL, "max_normal_angle must be in (0, 90) degrees");
     return VSM_OK;
@@ -455,14 +457,17 @@ int check_params(vsm_ctx *c, const vsm_params &p)
 // changes a result.
 constexpr int kBlockTiles = 8;
 
-void tiles_for(const ViewDesc &V, int view_index, std::vector<Tile> &out)
+// Tiles of width tile_w and height 2 kThreads / tile_w (the half-step kernel's
+// shape; the filter and cost kernels use kTileW x kTileH).
+void tiles_for(const ViewDesc &V, int view_index, std::vector<Tile> &out, int tile_w = kTileW)
 {
-    const int tw = (V.W + kTileW - 1) / kTileW, th = (V.H + kTileH - 1) / kTileH;
+    const int tile_h = 2 * kThreads / tile_w;
+    const int tw = (V.W + tile_w - 1) / tile_w, th = (V.H + tile_h - 1) / tile_h;
     for (int by = 0; by < th; by += kBlockTiles)
         for (int bx = 0; bx < tw; bx += kBlockTiles)
             for (int ty = by; ty < std::min(th, by + kBlockTiles); ++ty)
                 for (int tx = bx; tx < std::min(tw, bx + kBlockTiles); ++tx)
-                    out.push_back(Tile{view_index, tx * kTileW, ty * kTileH, 0});
+                    out.push_back(Tile{view_index, tx * tile_w, ty * tile_h, 0});
 }
 
 size_t state_bytes(int W, int H, int S)
@@ -545,10 +550,10 @@ void release_view(vsm_ctx *c, ViewState &vs)
     vs.hyp = nullptr; vs.cost = nullptr; vs.prob = nullptr;
 }
 
-int grid_for(vsm_ctx *c, int maxs)
+int grid_for(vsm_ctx *c, int maxs, int tw)
 {
-    const int k = maxs <= 8 ? 0 : (maxs <= 16 ? 1 : 2);
-    if (!c->grid[k]) c->grid[k] = halfstep_grid(maxs);
+    const int k = (maxs <= 8 ? 0 : (maxs <= 16 ? 1 : 2)) * 3 + (tw == 16 ? 0 : (tw == 32 ? 1 : 2));
+    if (!c->grid[k]) c->grid[k] = halfstep_grid(maxs, tw);
     return c->grid[k];
 }
 
@@ -556,10 +561,10 @@ int grid_for(vsm_ctx *c, int maxs)
 // view-selection costs (32 sources x 256 threads) in global memory; sized for
 // the largest grid it is launched with.  Called before any launch (and before
 // graph capture).
-int ensure_c0(vsm_ctx *c, int maxs)
+int ensure_c0(vsm_ctx *c, int maxs, int tw)
 {
     if (maxs <= 16) return VSM_OK;
-    const size_t need = (size_t)grid_for(c, maxs) * 32 * 256;
+    const size_t need = (size_t)grid_for(c, maxs, tw) * 32 * 256;
     if (c->c0_floats >= need) return VSM_OK;
     if (c->d_c0) dfree(c, c->d_c0);
     c->d_c0 = nullptr;
@@ -570,16 +575,16 @@ int ensure_c0(vsm_ctx *c, int maxs)
     return VSM_OK;
 }
 
-int record_halfstep(vsm_ctx *c, int maxs, const ViewDesc *dviews, const Tile *dtiles, int ntiles,
+int record_halfstep(vsm_ctx *c, int maxs, int tw, const ViewDesc *dviews, const Tile *dtiles, int ntiles,
                     const PMConst &pc, bool timed)
 {
     {
-        const int rc = ensure_c0(c, maxs);
+        const int rc = ensure_c0(c, maxs, tw);
         if (rc) return rc;
     }
     if (timed) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    HIPCHK(c, launch_halfstep(maxs, dviews, dtiles, ntiles, pc, c->d_counters, c->d_work,
-                              grid_for(c, maxs), c->d_c0, c->stream));
+    HIPCHK(c, launch_halfstep(maxs, tw, dviews, dtiles, ntiles, pc, c->d_counters, c->d_work,
+                              grid_for(c, maxs, tw), c->d_c0, c->stream));
     if (timed) {
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
         HIPCHK(c, hipEventSynchronize(c->ev1));
@@ -686,15 +691,20 @@ int batch_begin(vsm_ctx *c, const vsm_view_req *reqs, int n, int pass, int use_g
     std::vector<ViewDesc> hv(n);
     for (int v = 0; v < n; ++v) {
         b.maxs = std::max(b.maxs, b.views[v].desc.n_src);
-        tiles_for(b.views[v].desc, v, tiles);
+        tiles_for(b.views[v].desc, v, tiles, b.params.tile_width);
         b.ref_ids.push_back(reqs[v].ref_id);
         hv[v] = b.views[v].desc;
     }
     b.ntiles = (int)tiles.size();
     b.nsteps = 2 * b.params.num_iterations;
+    if (!tile_width_supported(b.params.tile_width, b.maxs)) {
+        batch_release(c, b);
+        return fail(c, VSM_EINVAL, "tile_width %d is not available for %d sources (64-wide tiles take at "
+                    "most 16)", b.params.tile_width, b.maxs);
+    }
     rc = dalloc(c, &b.dviews, n);
     if (!rc) rc = dalloc(c, &b.dtiles, tiles.size());
-    if (!rc) rc = ensure_c0(c, b.maxs);     // before any capture
+    if (!rc) rc = ensure_c0(c, b.maxs, b.params.tile_width);     // before any capture
     if (rc) {
         batch_release(c, b);
         return rc;
@@ -712,7 +722,7 @@ int batch_begin(vsm_ctx *c, const vsm_view_req *reqs, int n, int pass, int use_g
     BEGINCHK(hipMemcpyAsync(b.dtiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice,
                             c->stream));
     if (pass == 0) {
-        BEGINCHK(launch_init(b.dviews, b.dtiles, b.ntiles, pc, 1, c->stream));
+        BEGINCHK(launch_init(b.dviews, b.dtiles, b.ntiles, pc, 1, b.params.tile_width, c->stream));
     } else {
         for (int v = 0; v < n; ++v) {
             const vsm_view_req &r = reqs[v];
@@ -739,7 +749,7 @@ int batch_begin(vsm_ctx *c, const vsm_view_req *reqs, int n, int pass, int use_g
                             "(solve pass 0 first, set them, or pass init_depth/init_normal)", r.ref_id);
             }
         }
-        BEGINCHK(launch_init(b.dviews, b.dtiles, b.ntiles, pc, 0, c->stream));
+        BEGINCHK(launch_init(b.dviews, b.dtiles, b.ntiles, pc, 0, b.params.tile_width, c->stream));
     }
     BEGINCHK(hipStreamSynchronize(c->stream));
 #undef BEGINCHK
@@ -763,8 +773,9 @@ int batch_step(vsm_ctx *c, Batch &b, int k)
             HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             hipError_t ce = hipSuccess;
             for (int s = 0; s < m && ce == hipSuccess; ++s) {
-                ce = launch_halfstep(b.maxs, b.dviews, b.dtiles, b.ntiles, pc, c->d_counters, c->d_work,
-                                     grid_for(c, b.maxs), c->d_c0, c->stream);
+                ce = launch_halfstep(b.maxs, b.params.tile_width, b.dviews, b.dtiles, b.ntiles, pc,
+                                     c->d_counters, c->d_work, grid_for(c, b.maxs, b.params.tile_width),
+                                     c->d_c0, c->stream);
                 if (ce == hipSuccess) ce = launch_advance(b.dviews, (int)b.views.size(), c->stream);
             }
             const hipError_t ee = hipStreamEndCapture(c->stream, &graph);
@@ -791,7 +802,8 @@ int batch_step(vsm_ctx *c, Batch &b, int k)
         c->halfstep_launches += m;
     } else {
         for (int s = 0; s < m; ++s) {
-            const int rc = record_halfstep(c, b.maxs, b.dviews, b.dtiles, b.ntiles, pc, c->timing);
+            const int rc = record_halfstep(c, b.maxs, b.params.tile_width, b.dviews, b.dtiles, b.ntiles, pc,
+                                           c->timing);
             if (rc) return rc;
             HIPCHK(c, launch_advance(b.dviews, (int)b.views.size(), c->stream));
         }
@@ -883,6 +895,7 @@ void vsm_default_params(vsm_params *p)
     p->n_far = 6;
     p->seed = 0x5A7E11173Dull;
     p->draw_group = 0;
+    p->tile_width = 32;
 }
 
 int vsm_create(int device, uint64_t hbm_budget_bytes, vsm_ctx **out)
@@ -1175,15 +1188,20 @@ static int debug_state(vsm_ctx *c, const vsm_view_req *req, int pass, int mode, 
     const size_t hw = (size_t)V.W * V.H;
     V.step = 2 * iter + color;
     V.num_steps = V.step + 1;
+    const int tw = c->params.tile_width;
+    if (mode == 1 && !tile_width_supported(tw, V.n_src)) {
+        release_view(c, vs);
+        return fail(c, VSM_EINVAL, "tile_width %d is not available for %d sources", tw, V.n_src);
+    }
     std::vector<Tile> tiles;
-    tiles_for(V, 0, tiles);
+    tiles_for(V, 0, tiles, mode == 2 ? kTileW : tw);
     ViewDesc *dv = nullptr;
     Tile *dt = nullptr;
     uint32_t *dm = nullptr;
     rc = dalloc(c, &dv, 1);
     if (!rc) rc = dalloc(c, &dt, tiles.size());
     if (!rc && mode == 2) rc = dalloc(c, &dm, hw);
-    if (!rc && mode == 1) rc = ensure_c0(c, V.n_src);
+    if (!rc && mode == 1) rc = ensure_c0(c, V.n_src, tw);
     const PMConst pc = make_const(c->params);
     std::vector<float4> h(hw);
     if (!rc) {
@@ -1197,10 +1215,10 @@ static int debug_state(vsm_ctx *c, const vsm_view_req *req, int pass, int mode, 
         HIPCHK(c, hipMemcpyAsync(dv, &V, sizeof V, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(dt, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream));
         const int nt = (int)tiles.size();
-        if (mode == 0) HIPCHK(c, launch_init(dv, dt, nt, pc, 1, c->stream));
+        if (mode == 0) HIPCHK(c, launch_init(dv, dt, nt, pc, 1, tw, c->stream));
         else if (mode == 1)
-            HIPCHK(c, launch_halfstep(V.n_src, dv, dt, nt, pc, c->d_counters, c->d_work,
-                                      grid_for(c, V.n_src), c->d_c0, c->stream));
+            HIPCHK(c, launch_halfstep(V.n_src, tw, dv, dt, nt, pc, c->d_counters, c->d_work,
+                                      grid_for(c, V.n_src, tw), c->d_c0, c->stream));
         else HIPCHK(c, launch_filter(V.n_src, dv, dt, nt, pc, dm, c->stream));
         HIPCHK(c, hipMemcpyAsync(h.data(), vs.hyp, hw * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
         if (cost && mode != 2) HIPCHK(c, hipMemcpyAsync(cost, vs.cost, hw * 4, hipMemcpyDeviceToHost, c->stream));
@@ -1600,7 +1618,8 @@ int vsm_median_blur3(vsm_ctx *c, const float *in, int32_t e_size, int32_t n_size
 // and the init's tile lists and descriptors, and block rounding.
 static size_t serve_scratch_bytes(int W, int H)
 {
-    const size_t ntiles = (size_t)((W + kTileW - 1) / kTileW) * ((H + kTileH - 1) / kTileH);
+    // tile counts of the narrowest half-step tiles (16 x 32) bound every width's
+    const size_t ntiles = (size_t)((W + 15) / 16) * ((H + 31) / 32) + (size_t)((W + 63) / 64) * ((H + 7) / 8);
     return 4 * ntiles * sizeof(Tile) + 2 * sizeof(ViewDesc) + 16 * kAllocAlign;
 }
 
@@ -1702,7 +1721,8 @@ static int serve_start(vsm_ctx *c, ServeReq &r)
             e = hipMemcpyAsync(dt, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream);
         if (e == hipSuccess && r.pass == 1)     // start from the photometric depth and normals
             e = launch_load_hyp(dv, 0, (int)hw, img.depth[0], img.normal[0], pc.cos_max_normal, c->stream);
-        if (e == hipSuccess) e = launch_init(dv, dt, (int)tl.size(), pc, r.pass == 0 ? 1 : 0, c->stream);
+        if (e == hipSuccess)
+            e = launch_init(dv, dt, (int)tl.size(), pc, r.pass == 0 ? 1 : 0, kTileW, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) rc = fail(c, VSM_EDEVICE, "request start failed: %s", hipGetErrorString(e));
     }
@@ -1827,7 +1847,10 @@ static int serve_submit(vsm_ctx *c, const vsm_view_req *req, int pass, int filte
     // resident serving scratch is allocated before any admission decision
     int rc = VSM_OK;
     if (!c->serve_views && (rc = dalloc(c, &c->serve_views, kServeMaxViews))) return rc;
-    if ((rc = ensure_c0(c, req->n_src))) return rc;
+    if (!tile_width_supported(c->params.tile_width, req->n_src))
+        return fail(c, VSM_EINVAL, "tile_width %d is not available for %d sources", c->params.tile_width,
+                    req->n_src);
+    if ((rc = ensure_c0(c, req->n_src, c->params.tile_width))) return rc;
     auto r = std::make_unique<ServeReq>();
     r->ref = req->ref_id;
     r->pass = pass;
@@ -1871,7 +1894,7 @@ int vsm_serve_step(vsm_ctx *c, int32_t *n_active, int32_t *n_queued)
             hv[v] = r.vs.desc;
             hv[v].step = r.host_step;
             maxs = std::max(maxs, hv[v].n_src);
-            tiles_for(hv[v], v, tiles);
+            tiles_for(hv[v], v, tiles, c->params.tile_width);
         }
         if ((int)tiles.size() > c->serve_cap_tiles) {
             dfree(c, c->serve_tiles);
@@ -1880,10 +1903,14 @@ int vsm_serve_step(vsm_ctx *c, int32_t *n_active, int32_t *n_queued)
             if ((rc = dalloc(c, &c->serve_tiles, tiles.size() * 2))) return rc;
             c->serve_cap_tiles = (int)tiles.size() * 2;
         }
+        if (!tile_width_supported(c->params.tile_width, maxs))
+            return fail(c, VSM_EINVAL, "tile_width %d is not available for %d sources", c->params.tile_width,
+                        maxs);
         const PMConst pc = make_const(c->params);
         HIPCHK(c, hipMemcpyAsync(c->serve_views, hv.data(), na * sizeof(ViewDesc), hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->serve_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream));
-        if ((rc = record_halfstep(c, maxs, c->serve_views, c->serve_tiles, (int)tiles.size(), pc, c->timing)))
+        if ((rc = record_halfstep(c, maxs, c->params.tile_width, c->serve_views, c->serve_tiles,
+                                  (int)tiles.size(), pc, c->timing)))
             return rc;
         HIPCHK(c, hipStreamSynchronize(c->stream));
         // Every active request has advanced one half-step.  The finished ones

@@ -33,7 +33,8 @@ struct PMConst {
     float perturbation, normal_perturbation, cos_max_normal, one_minus_cos_max, decay_base;
     int n_far;
     uint32_t seed_lo, seed_hi;
-    int draw_group;                 // Monte-Carlo draws keyed by 32x4-pixel group (1) or pixel (0)
+    int draw_group;                 // Monte-Carlo draws keyed by pixel (0) or by the g x 128/g
+                                    // pixel group of one wave of a g-wide tile (g = draw_group)
 };
 
 // One reference view and its sources (device memory; read through scalar loads).

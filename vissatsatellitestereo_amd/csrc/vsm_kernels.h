@@ -5,13 +5,16 @@
 
 namespace vsm {
 
-hipError_t launch_halfstep(int maxs, const ViewDesc *views, const Tile *tiles, int ntiles,
+// Half-step kernel for up to maxs sources on tiles of tw x (512 / tw)
+// pixels: 16 or 32 wide, and 64 for at most 16 sources.
+bool tile_width_supported(int tw, int maxs);
+hipError_t launch_halfstep(int maxs, int tw, const ViewDesc *views, const Tile *tiles, int ntiles,
                            const PMConst &c, Counters *counters, unsigned int *work, int grid,
                            float *c0_scratch, hipStream_t st);
 // Resident workgroups of the persistent half-step kernel on this device.
-int halfstep_grid(int maxs);
+int halfstep_grid(int maxs, int tw);
 hipError_t launch_init(const ViewDesc *views, const Tile *tiles, int ntiles, const PMConst &c,
-                       int random_hyp, hipStream_t st);
+                       int random_hyp, int tw, hipStream_t st);
 hipError_t launch_eval_cost(int maxs, const ViewDesc *views, const Tile *tiles, int ntiles,
                             const PMConst &c, float *out, hipStream_t st);
 hipError_t launch_filter(int maxs, const ViewDesc *views, const Tile *tiles, int ntiles,

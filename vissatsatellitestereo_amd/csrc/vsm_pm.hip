@@ -167,6 +167,7 @@ __device__ __forceinline__ float tap_weight(const PMConst &c, float r, int dx, i
 struct RefStats {
     float rc, sw, inv_sw, mr, var_r;
     const float *wt;             // tap weights in LDS (stride kThreads), or null
+    int ldsw;                    // row pitch of the reference tile in LDS
 };
 
 // Windows of at most kWTab taps keep each pixel's tap weights in LDS (written
@@ -174,17 +175,18 @@ struct RefStats {
 // 36 KB per workgroup for 6 x 6 taps).
 constexpr int kWTab = 36;
 
-__device__ __forceinline__ RefStats ref_stats(const PMConst &c, const float *s_ref, int lbase,
+__device__ __forceinline__ RefStats ref_stats(const PMConst &c, const float *s_ref, int lbase, int ldsw,
                                               float *wt = nullptr)
 {
     RefStats st;
     st.rc = s_ref[lbase];
     st.wt = wt;
+    st.ldsw = ldsw;
     float sw = 0.f, sr = 0.f, srr = 0.f;
     int t = 0;
     for (int j = 0; j < c.ntaps; ++j) {
         const int dy = -c.radius + j * c.step;
-        const float *row = s_ref + lbase + dy * kLdsW;
+        const float *row = s_ref + lbase + dy * ldsw;
         for (int i = 0; i < c.ntaps; ++i, ++t) {
             const int dx = -c.radius + i * c.step;
             const float r = row[dx] - st.rc;
@@ -310,7 +312,7 @@ __device__ __forceinline__ void window_pass(const PMConst &c, const SrcView (&SV
     int ic = 0, jc = 0, tc = 0;               // next tap to accumulate
     auto consume = [&](const uint32_t (&v)[G], const float (&fx)[G], const float (&fy)[G]) {
         const int dx = -rad + ic * st, dy = -rad + jc * st;
-        const float r = s_ref[lbase + dy * kLdsW + dx] - R.rc;
+        const float r = s_ref[lbase + dy * R.ldsw + dx] - R.rc;
         const float w = WT ? R.wt[tc * kThreads] : tap_weight(c, r, dx, dy);
         ++tc;
         const float wr = w * r;
@@ -569,45 +571,57 @@ __device__ __forceinline__ void eval_hyps(const PMConst &c, const SrcLds &Ss, fl
     }
 }
 
+// Tile shapes: TW x TH pixels with TW * TH = 2 kThreads, so in a red-black
+// half-step each thread owns one pixel of the active colour; a wave covers
+// 128 / TW rows of the tile.  The reference tile with its halo sits in LDS
+// with row pitch TW + 2 kMaxRadius.
+template <int TW> struct TileShape {
+    static constexpr int W = TW, H = 2 * kThreads / TW;
+    static constexpr int LW = TW + 2 * kMaxRadius, LH = H + 2 * kMaxRadius;
+};
+
 // Common tile prologue: reference tile (replicated border) into LDS and the
 // tile-centred source matrices.
-template <int MAXS>
+template <int MAXS, int TW>
 __device__ __forceinline__ void tile_prologue(const ViewDesc &V, const Tile &t, float *s_ref,
                                               SrcLds *s_src)
 {
+    using T = TileShape<TW>;
     const int W = V.W, H = V.H;
-    for (int k = threadIdx.x; k < kLdsH * kLdsW; k += kThreads) {
-        const int ly = k / kLdsW, lx = k - ly * kLdsW;
+    for (int k = threadIdx.x; k < T::LH * T::LW; k += kThreads) {
+        const int ly = k / T::LW, lx = k - ly * T::LW;
         const int gx = min(max(t.x0 + lx - kMaxRadius, 0), W - 1);
         const int gy = min(max(t.y0 + ly - kMaxRadius, 0), H - 1);
         s_ref[k] = (float)V.ref[gy * W + gx];
     }
     if ((int)threadIdx.x < V.n_src)
-        setup_source(V, threadIdx.x, t.x0 + kTileW / 2, t.y0 + kTileH / 2, s_src[threadIdx.x]);
+        setup_source(V, threadIdx.x, t.x0 + T::W / 2, t.y0 + T::H / 2, s_src[threadIdx.x]);
 }
 
 // One red-black half-step on one tile (all 256 threads of the workgroup
 // call it; it contains one barrier).
-template <int MAXS>
+template <int MAXS, int TW>
 __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, const PMConst &c,
                                               float *s_ref, SrcLds *s_src, float *s_c0,
                                               uint32_t *s_list, float *s_w, unsigned int &my_ncc,
                                               unsigned int &my_geo)
 {
+    using T = TileShape<TW>;
     const int iter = V.step >> 1, color = V.step & 1, pass = V.pass;
     const int W = V.W, H = V.H, S = V.n_src;
     const int tid = threadIdx.x;
-    tile_prologue<MAXS>(V, t, s_ref, s_src);
+    tile_prologue<MAXS, TW>(V, t, s_ref, s_src);
     __syncthreads();
 
-    const int ty = tid >> 4;
-    const int tx = ((tid & 15) << 1) + ((t.y0 + ty + color) & 1);
+    constexpr int kHalf = TW / 2;               // active pixels per tile row
+    const int ty = tid / kHalf;
+    const int tx = ((tid % kHalf) << 1) + ((t.y0 + ty + color) & 1);
     const int x = t.x0 + tx, y = t.y0 + ty;
     if (x < W && y < H) {
         const size_t HW = (size_t)W * H;
         const int p = y * W + x;
-        const float du = (float)(tx - kTileW / 2), dv = (float)(ty - kTileH / 2);
-        const int lbase = (ty + kMaxRadius) * kLdsW + tx + kMaxRadius;
+        const float du = (float)(tx - T::W / 2), dv = (float)(ty - T::H / 2);
+        const int lbase = (ty + kMaxRadius) * T::LW + tx + kMaxRadius;
 
         // ---- candidate hypotheses ---------------------------------------
         float hd[kNHyp], hn0[kNHyp], hn1[kNHyp], hn2[kNHyp];
@@ -668,7 +682,7 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
 
         // ---- view selection on the current hypothesis --------------------
         const bool wtab = c.ntaps * c.ntaps <= kWTab;
-        const RefStats R = ref_stats(c, s_ref, lbase, wtab ? s_w + tid : nullptr);
+        const RefStats R = ref_stats(c, s_ref, lbase, T::LW, wtab ? s_w + tid : nullptr);
         const float a0 = -hn0[0] / hn2[0], b0 = -hn1[0] / hn2[0];
         const float nh0[3] = {hn0[0], hn1[0], hn2[0]};
         float jtn;
@@ -769,10 +783,14 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
             }
         } else {
             float tk[kMaxSamples];
-            // the draws' counter word: the pixel, or its 32 x 4 group (one
-            // wave's pixels of a tile) whose pixels share the uniforms
-            const uint32_t dkey = c.draw_group ? (uint32_t)((y >> 2) * ((W + 31) >> 5) + (x >> 5))
-                                               : (uint32_t)p;
+            // the draws' counter word: the pixel, or its group of
+            // g x 128 / g pixels (g = c.draw_group, the tile width: the
+            // pixels one wave covers), which share the uniforms
+            uint32_t dkey = (uint32_t)p;
+            if (c.draw_group) {
+                const int gw = c.draw_group, gh = 128 / gw;
+                dkey = (uint32_t)((y / gh) * ((W + gw - 1) / gw) + x / gw);
+            }
 #pragma unroll
             for (int b = 0; b < kMaxSamples / 4; ++b) {
                 float u[4];
@@ -865,13 +883,13 @@ constexpr int kXcd = 8;   // MI355X: 8 XCDs, workgroups dispatched round-robin a
 // XCDs' shares.  work[8] are per-XCD cursors zeroed before the launch.  The
 // order tiles are taken in never changes a result: tiles are independent
 // within a half-step.
-template <int MAXS>
+template <int MAXS, int TW>
 __global__ __launch_bounds__(kThreads, 2) void k_halfstep(const ViewDesc *__restrict__ views,
                                                        const Tile *__restrict__ tiles, int ntiles,
                                                        PMConst c, Counters *counters,
                                                        unsigned int *work, float *c0g)
 {
-    __shared__ float s_ref[kLdsH * kLdsW];
+    __shared__ float s_ref[TileShape<TW>::LH * TileShape<TW>::LW];
     __shared__ SrcLds s_src[MAXS];
     // kernels for more than 16 sources keep the view-selection costs in a
     // per-workgroup global scratch, which leaves LDS room for the tap weights
@@ -900,7 +918,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_halfstep(const ViewDesc *__rest
             const Tile t = tiles[ti];
             const ViewDesc &V = views[t.view];
             if (!V.active || V.step >= V.num_steps) continue;     // uniform per workgroup
-            halfstep_tile<MAXS>(V, t, c, s_ref, s_src,
+            halfstep_tile<MAXS, TW>(V, t, c, s_ref, s_src,
                                 MAXS > 16 ? c0g + (size_t)blockIdx.x * (MAXS * kThreads) : s_c0,
                                 s_list, s_w, my_ncc, my_geo);
         }
@@ -916,16 +934,17 @@ __global__ __launch_bounds__(kThreads, 2) void k_halfstep(const ViewDesc *__rest
     }
 }
 
-// Random initialisation: uniform depth in the range, normal in the cone.
+// Random initialisation: uniform depth in the range, normal in the cone
+// (tiles of tw x 2 kThreads / tw pixels).
 __global__ __launch_bounds__(kThreads) void k_init(const ViewDesc *__restrict__ views,
                                                    const Tile *__restrict__ tiles, PMConst c,
-                                                   int random_hyp)
+                                                   int random_hyp, int tw)
 {
     const Tile t = tiles[blockIdx.x];
     const ViewDesc &V = views[t.view];
     const size_t HW = (size_t)V.W * V.H;
-    for (int k = threadIdx.x; k < kTileW * kTileH; k += kThreads) {
-        const int x = t.x0 + (k & (kTileW - 1)), y = t.y0 + k / kTileW;
+    for (int k = threadIdx.x; k < 2 * kThreads; k += kThreads) {
+        const int x = t.x0 + k % tw, y = t.y0 + k / tw;
         if (x >= V.W || y >= V.H) continue;
         const int p = y * V.W + x;
         if (random_hyp) {
@@ -949,7 +968,7 @@ __global__ __launch_bounds__(kThreads) void k_eval_cost(const ViewDesc *__restri
     __shared__ SrcLds s_src[MAXS];
     const Tile t = tiles[blockIdx.x];
     const ViewDesc &V = views[t.view];
-    tile_prologue<MAXS>(V, t, s_ref, s_src);
+    tile_prologue<MAXS, kTileW>(V, t, s_ref, s_src);
     __syncthreads();
     const size_t HW = (size_t)V.W * V.H;
     for (int k = threadIdx.x; k < kTileW * kTileH; k += kThreads) {
@@ -959,7 +978,7 @@ __global__ __launch_bounds__(kThreads) void k_eval_cost(const ViewDesc *__restri
         const int p = y * V.W + x;
         const int lbase = (ty + kMaxRadius) * kLdsW + tx + kMaxRadius;
         const float du = (float)(tx - kTileW / 2), dv = (float)(ty - kTileH / 2);
-        const RefStats R = ref_stats(c, s_ref, lbase);
+        const RefStats R = ref_stats(c, s_ref, lbase, kLdsW);
         const float4 h = V.hyp[p];
         const float a = -h.y / h.w, b = -h.z / h.w;
         for (int s = 0; s < V.n_src; ++s) {
@@ -1013,7 +1032,7 @@ __global__ __launch_bounds__(kThreads) void k_filter(const ViewDesc *__restrict_
     const Tile t = tiles[blockIdx.x];
     const ViewDesc &V = views[t.view];
     const bool wtab = c.ntaps * c.ntaps <= kWTab;
-    tile_prologue<MAXS>(V, t, s_ref, s_src);
+    tile_prologue<MAXS, kTileW>(V, t, s_ref, s_src);
     __syncthreads();
     for (int k = threadIdx.x; k < kTileW * kTileH; k += kThreads) {
         const int tx = k & (kTileW - 1), ty = k / kTileW;
@@ -1030,7 +1049,7 @@ __global__ __launch_bounds__(kThreads) void k_filter(const ViewDesc *__restrict_
         {
             const int lbase = (ty + kMaxRadius) * kLdsW + tx + kMaxRadius;
             const float du = (float)(tx - kTileW / 2), dv = (float)(ty - kTileH / 2);
-            const RefStats R = ref_stats(c, s_ref, lbase, wtab ? s_w + threadIdx.x : nullptr);
+            const RefStats R = ref_stats(c, s_ref, lbase, kLdsW, wtab ? s_w + threadIdx.x : nullptr);
             const float a = -h.y / h.w, b = -h.z / h.w;
             // up to four sources share a window pass (as in view selection)
             const int S = V.n_src;
@@ -1151,43 +1170,68 @@ __global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_halfstep(int maxs, const ViewDesc *views, const Tile *tiles, int ntiles,
+// The kernel instantiation for (sources, tile width); null for an
+// unsupported width.
+typedef void (*halfstep_fn)(const ViewDesc *, const Tile *, int, PMConst, Counters *, unsigned int *,
+                            float *);
+
+template <int MAXS>
+static halfstep_fn halfstep_for_width(int tw)
+{
+    switch (tw) {
+    case 16: return k_halfstep<MAXS, 16>;
+    case 32: return k_halfstep<MAXS, 32>;
+    default: return tw == 64 ? k_halfstep<MAXS, 64> : nullptr;
+    }
+}
+
+// 64-wide tiles need 1.5 KB more LDS for the reference tile than the kernel
+// for more than 16 sources has at 2 workgroups per CU: that kernel has 16-
+// and 32-wide tiles only.
+static halfstep_fn halfstep_kernel(int maxs, int tw)
+{
+    if (maxs <= 8) return halfstep_for_width<8>(tw);
+    if (maxs <= 16) return halfstep_for_width<16>(tw);
+    if (tw == 16) return k_halfstep<32, 16>;
+    if (tw == 32) return k_halfstep<32, 32>;
+    return nullptr;
+}
+
+bool tile_width_supported(int tw, int maxs) { return tw == 16 || tw == 32 || (tw == 64 && maxs <= 16); }
+
+hipError_t launch_halfstep(int maxs, int tw, const ViewDesc *views, const Tile *tiles, int ntiles,
                            const PMConst &c, Counters *counters, unsigned int *work, int grid,
                            float *c0_scratch, hipStream_t st)
 {
     if (ntiles <= 0) return hipSuccess;
+    const halfstep_fn fn = halfstep_kernel(maxs, tw);
+    if (!fn) return hipErrorInvalidValue;
     // more than 16 sources: c0_scratch holds grid x 32 x kThreads floats
     if (maxs > 16 && !c0_scratch) return hipErrorInvalidValue;
     hipError_t e = launch_fill_u32(work, kXcd, 0u, st);
     if (e != hipSuccess) return e;
     grid = max(kXcd, min(grid, ((ntiles + kXcd - 1) / kXcd) * kXcd));
-    if (maxs <= 8)
-        k_halfstep<8><<<grid, kThreads, 0, st>>>(views, tiles, ntiles, c, counters, work, c0_scratch);
-    else if (maxs <= 16)
-        k_halfstep<16><<<grid, kThreads, 0, st>>>(views, tiles, ntiles, c, counters, work, c0_scratch);
-    else
-        k_halfstep<32><<<grid, kThreads, 0, st>>>(views, tiles, ntiles, c, counters, work, c0_scratch);
+    fn<<<grid, kThreads, 0, st>>>(views, tiles, ntiles, c, counters, work, c0_scratch);
     return hipGetLastError();
 }
 
-int halfstep_grid(int maxs)
+int halfstep_grid(int maxs, int tw)
 {
     int dev = 0, ncu = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 1024;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) ncu = 256;
-    hipError_t e;
-    if (maxs <= 8) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_halfstep<8>, kThreads, 0);
-    else if (maxs <= 16) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_halfstep<16>, kThreads, 0);
-    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_halfstep<32>, kThreads, 0);
+    const halfstep_fn fn = halfstep_kernel(maxs, tw);
+    hipError_t e = fn ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kThreads, 0)
+                      : hipErrorInvalidValue;
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
     return ((ncu * per_cu + kXcd - 1) / kXcd) * kXcd;
 }
 
 hipError_t launch_init(const ViewDesc *views, const Tile *tiles, int ntiles, const PMConst &c,
-                       int random_hyp, hipStream_t st)
+                       int random_hyp, int tw, hipStream_t st)
 {
     if (ntiles <= 0) return hipSuccess;
-    k_init<<<ntiles, kThreads, 0, st>>>(views, tiles, c, random_hyp);
+    k_init<<<ntiles, kThreads, 0, st>>>(views, tiles, c, random_hyp, tw);
     return hipGetLastError();
 }
 

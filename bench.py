@@ -436,6 +436,40 @@ def workload(cfg):
     return spec, max_src, counts, res_m, desc
 
 
+def lds_staging_ab(a, cfg):
+    """A/B of the window-tap gather path: every view's cost of its converged
+    photometric depths (fronto-parallel planes) against every source, the taps
+    read through the caches (A) or from a per-(tile, source) box of the source
+    staged in LDS (B, where the box fits 4096 texels).  Same costs either way."""
+    from vissatsatellitestereo_amd import runtime  # noqa: F401
+    spec, max_src, counts, res_m, desc = workload(cfg)
+    work = build_site(spec, os.path.join(a.work, cfg), max_src, counts, res_m)
+    site = load_site(work)
+    ctx = make_ctx([site], a)
+    views = site["views"][:a.ab_views]
+    out = {"workload": f"{cfg}: {desc}; {len(views)} views, r={a.window_radius}", "views": []}
+    for ws in (1, 2):
+        ctx.set_params(num_iterations=a.iters, window_step=ws, geom_consistency=0, filter=0)
+        res = ctx.solve(views, 0, want=("depth",), use_graph=True)
+        tot = {0: 0.0, 1: 0.0}
+        for v, r in zip(views, res):
+            n = np.zeros(r["depth"].shape + (3,), np.float32)
+            n[..., 2] = 1.0
+            c0, t0, _ = ctx.eval_cost_staging(v["ref"], v["srcs"], v["depth_range"], r["depth"], n, 0, a.ab_repeats)
+            c1, t1, frac = ctx.eval_cost_staging(v["ref"], v["srcs"], v["depth_range"], r["depth"], n, 1,
+                                                  a.ab_repeats)
+            same = bool(np.array_equal(c0, c1))
+            tot[0] += t0
+            tot[1] += t1
+            out["views"].append({"ref": v["ref"], "window_step": ws, "sources": len(v["srcs"]),
+                                 "ms_global": round(t0, 3), "ms_lds": round(t1, 3),
+                                 "staged_fraction": round(frac, 4), "identical_costs": same})
+            log(out["views"][-1])
+        out[f"step{ws}_lds_over_global_time"] = round(tot[1] / tot[0], 4)
+    ctx.close()
+    return out
+
+
 def oracle_bar(cfg):
     f = os.path.join(ROOT, "tests", "golden", f"{cfg}_oracle_dsm.json")
     if not os.path.exists(f):
@@ -760,11 +794,18 @@ def main():
     ap.add_argument("--serve-views", type=int, default=17, help="c4 views per site")
     ap.add_argument("--serve-max-side", type=int, default=3072)
     ap.add_argument("--work", default=os.environ.get("VSM_BENCH_WORK", "/tmp/vsm_bench"))
+    ap.add_argument("--lds-staging-ab", action="store_true",
+                    help="measure LDS-staged against cached gathers of source footprints and exit")
+    ap.add_argument("--ab-views", type=int, default=4)
+    ap.add_argument("--ab-repeats", type=int, default=3)
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; runs on more GPUs are not part of it",
                           "n_gpus": a.gpus}))
         sys.exit(2)
+    if a.lds_staging_ab:
+        print(json.dumps(lds_staging_ab(a, a.config)))
+        return
     if a.config == "c4":
         out = bench_serving(a)
     else:

@@ -285,6 +285,17 @@ int vsm_poll(vsm_ctx *ctx, uint64_t ticket, vsm_view_out *out, int32_t *done);
 int vsm_eval_cost(vsm_ctx *ctx, const vsm_view_req *req, int32_t pass, const float *depth,
                   const float *normal_uvd, float *out);
 
+/* Measurement entry point (no counterpart in the reference): the same costs
+ * as vsm_eval_cost, sources outer, with the window taps read either from the
+ * source texture through the caches (stage 0) or from a box of the source
+ * staged in LDS by each workgroup for each source (stage 1; a box of more than
+ * 4096 texels falls back to the caches).  Runs the kernel `repeats` times and
+ * reports the average kernel time and the fraction of (tile, source) pairs
+ * that were staged; out as in vsm_eval_cost (identical for both stages). */
+int vsm_eval_cost_staging(vsm_ctx *ctx, const vsm_view_req *req, int32_t pass, const float *depth,
+                          const float *normal_uvd, int32_t stage, int32_t repeats, float *out,
+                          float *ms_per_launch, float *staged_fraction);
+
 /* One red-black half-step on host-provided state (in/out): depth H*W,
  * normal_uvd H*W*3, cost H*W, prob n_src*H*W.  Test entry point for
  * lockstep parity against the oracle. */

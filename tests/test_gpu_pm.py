@@ -189,6 +189,31 @@ def test_lockstep_geometric_18_sources(many):
         assert r["frac_cost_err"] <= 1e-3, r
 
 
+def test_lds_staged_costs_equal_cached(tiny):
+    """The measurement kernel that reads window taps from LDS-staged boxes of
+    the sources returns the same costs as its cached-gather twin and as
+    vsm_eval_cost, for near-true and random planes, stride 1 and 2."""
+    for step in (1, 2):
+        p = _params(window_radius=5, window_step=step)
+        ctx, ref, srcs = _setup(tiny, p)
+        gt = sites.gt_depth(tiny, ref).astype(np.float32)
+        d, n = _random_hyp(tiny, ref, seed=11)
+        mixed = d.copy()
+        mixed[1::2] = gt[1::2]
+        n[1::2] = [0, 0, 1]
+        fronto = np.zeros_like(n)
+        fronto[..., 2] = 1.0
+        for dd, nn, must_stage in ((gt, fronto, True), (mixed, n, False)):
+            args = (_id(tiny, ref), [_id(tiny, s) for s in srcs], tiny["dr"][ref], dd, nn)
+            a, _, fa = ctx.eval_cost_staging(*args, stage=0, repeats=1)
+            b, _, fb = ctx.eval_cost_staging(*args, stage=1, repeats=1)
+            g = ctx.eval_cost(*args, 0)
+            print("step", step, "staged fraction", fb)
+            assert fa == 0.0 and (fb > 0.5 or not must_stage)
+            assert np.array_equal(a, b)
+            assert np.abs(a - g).max() <= 1e-6
+
+
 def test_init_matches_oracle(c1):
     p = _params()
     ctx, ref, srcs = _setup(c1, p)

@@ -124,6 +124,9 @@ SIGNATURES = [
     ("vsm_batch_step", _c.c_int, [_c.c_void_p, _c.c_int32, _c.c_int32, P(_c.c_int32)]),
     ("vsm_batch_end", _c.c_int, [_c.c_void_p, _c.c_int32, P(ViewOut), _c.c_int32]),
     ("vsm_set_hbm_budget", _c.c_int, [_c.c_void_p, _c.c_uint64]),
+    ("vsm_eval_cost_staging", _c.c_int, [_c.c_void_p, P(ViewReq), _c.c_int32, P(_c.c_float),
+                                         P(_c.c_float), _c.c_int32, _c.c_int32, P(_c.c_float),
+                                         P(_c.c_float), P(_c.c_float)]),
     ("vsm_proj_to_grid_noalloc", _c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_int64, P(DsmGrid),
                                             _c.c_void_p, _c.c_void_p]),
     ("vsm_submit", _c.c_int, [_c.c_void_p, P(ViewReq), P(_c.c_uint64)]),

@@ -1176,6 +1176,57 @@ int vsm_eval_cost(vsm_ctx *c, const vsm_view_req *req, int32_t pass, const float
     return rc;
 }
 
+int vsm_eval_cost_staging(vsm_ctx *c, const vsm_view_req *req, int32_t pass, const float *depth,
+                          const float *normal, int32_t stage, int32_t repeats, float *out,
+                          float *ms_per_launch, float *staged_fraction)
+{
+    if (!c || !req || !depth || !normal || repeats < 1) return VSM_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    ViewState vs;
+    int rc = prepare_view(c, *req, pass, (uint32_t)req->ref_id, vs);
+    if (rc) return rc;
+    const ViewDesc &V = vs.desc;
+    const size_t hw = (size_t)V.W * V.H;
+    std::vector<float4> h(hw);
+    for (size_t i = 0; i < hw; ++i)
+        h[i] = make_float4(depth[i], normal[3 * i], normal[3 * i + 1], normal[3 * i + 2]);
+    std::vector<Tile> tiles;
+    tiles_for(V, 0, tiles);
+    ViewDesc *dv = nullptr;
+    Tile *dt = nullptr;
+    float *dout = nullptr;
+    unsigned int *dn = nullptr;
+    rc = dalloc(c, &dv, 1);
+    if (!rc) rc = dalloc(c, &dt, tiles.size());
+    if (!rc) rc = dalloc(c, &dout, hw * V.n_src);
+    if (!rc) rc = dalloc(c, &dn, 1);
+    if (!rc) {
+        const PMConst pc = make_const(c->params);
+        const int nt = (int)tiles.size();
+        HIPCHK(c, hipMemcpyAsync(vs.hyp, h.data(), hw * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(dv, &V, sizeof V, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(dt, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, launch_eval_stage(V.n_src, stage, dv, dt, nt, pc, dout, nullptr, c->stream));   // warm
+        HIPCHK(c, launch_fill_u32(dn, 1, 0u, c->stream));
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+        for (int r = 0; r < repeats; ++r)
+            HIPCHK(c, launch_eval_stage(V.n_src, stage, dv, dt, nt, pc, dout, r == 0 ? dn : nullptr, c->stream));
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+        HIPCHK(c, hipEventSynchronize(c->ev1));
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        unsigned int ns = 0;
+        HIPCHK(c, hipMemcpyAsync(out, dout, hw * V.n_src * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(&ns, dn, 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (ms_per_launch) *ms_per_launch = ms / repeats;
+        if (staged_fraction) *staged_fraction = (float)ns / (float)((size_t)nt * V.n_src);
+    }
+    dfree(c, dv); dfree(c, dt); dfree(c, dout); dfree(c, dn);
+    release_view(c, vs);
+    return rc;
+}
+
 static int debug_state(vsm_ctx *c, const vsm_view_req *req, int pass, int mode, int iter, int color,
                        float *depth, float *normal, float *cost, float *prob, uint32_t *mask)
 {

@@ -19,6 +19,12 @@ hipError_t launch_eval_cost(int maxs, const ViewDesc *views, const Tile *tiles, 
                             const PMConst &c, float *out, hipStream_t st);
 hipError_t launch_filter(int maxs, const ViewDesc *views, const Tile *tiles, int ntiles,
                          const PMConst &c, uint32_t *mask, hipStream_t st);
+// Cost of the stored hypotheses against every source, sources outer, the
+// window taps read from an LDS-staged box of the source (stage != 0, where the
+// box fits) or from the global texture; *n_staged counts staged (tile, source)
+// pairs.
+hipError_t launch_eval_stage(int maxs, int stage, const ViewDesc *views, const Tile *tiles, int ntiles,
+                             const PMConst &c, float *out, unsigned int *n_staged, hipStream_t st);
 hipError_t launch_finalize(const ViewDesc *views, int view, int npix, float *depth, float *normal,
                            float *cost, float *depth_keep, hipStream_t st);
 hipError_t launch_load_hyp(const ViewDesc *views, int view, int npix, const float *depth,

@@ -24,6 +24,8 @@
 // four directions), random and perturbed hypotheses, Monte-Carlo view
 // selection with triangulation/incident-angle priors, geometric
 // forward-backward term, filter.
+#include <climits>
+
 #include "vsm_device.h"
 #include "vsm_kernels.h"
 
@@ -113,6 +115,7 @@ __device__ __forceinline__ Proj make_proj(const SrcLds &S, float du, float dv, f
 
 typedef const __attribute__((address_space(1))) uint32_t *gquad_t;
 typedef const __attribute__((address_space(1))) float *gfloat_t;
+typedef const __attribute__((address_space(3))) uint32_t *lquad_t;
 
 // floor(x) as an integer in one instruction (x finite).
 __device__ __forceinline__ int cvt_floor(float x)
@@ -209,8 +212,10 @@ __device__ __forceinline__ RefStats ref_stats(const PMConst &c, const float *s_r
 // blend_quad interpolates the loaded texel (DN).  The texel index
 // iy*W + ix is formed with a signed 24-bit multiply-add on anchor-relative
 // integers plus the anchor's offset ay*W + ax (|relative coordinates| < 2^23,
-// image side <= 32768).
-template <bool EXACT, bool CLAMP>
+// image side <= 32768).  With LDSRC the texel comes from a box of the
+// source staged in LDS (S.W4 and S.base4 are then the box pitch and the
+// anchor's index in it, in texels).
+template <bool EXACT, bool CLAMP, bool LDSRC = false>
 __device__ __forceinline__ void tap_issue(const SrcView &S, const Proj &P, float giu, float rx, float ry,
                                           float rz, float fi, uint32_t &v, float &fx, float &fy)
 {
@@ -222,8 +227,12 @@ __device__ __forceinline__ void tap_issue(const SrcView &S, const Proj &P, float
     const int ix = cvt_floor(px), iy = cvt_floor(py);
     fx = __builtin_amdgcn_fractf(px);
     fy = __builtin_amdgcn_fractf(py);
-    const unsigned off = (unsigned)(__mul24(iy, S.W4) + ((ix << 2) + S.base4));
-    v = *(gquad_t)((const __attribute__((address_space(1))) char *)S.quad + off);
+    if constexpr (LDSRC) {
+        v = ((lquad_t)S.quad)[__mul24(iy, S.W4) + ix + S.base4];
+    } else {
+        const unsigned off = (unsigned)(__mul24(iy, S.W4) + ((ix << 2) + S.base4));
+        v = *(gquad_t)((const __attribute__((address_space(1))) char *)S.quad + off);
+    }
 }
 
 __device__ __forceinline__ float blend_quad(uint32_t v, float fx, float fy)
@@ -262,7 +271,7 @@ __device__ __forceinline__ void row_start(const Proj (&P)[G], const float (&iz0)
 // weight and the reference value of each tap are computed once and shared;
 // the G gathers per tap are independent.  EXACT selects the per-tap
 // reciprocal (fallback when some lane's window exceeds kDeltaMax).
-template <int G, bool EXACT, bool CLAMP, bool WT>
+template <int G, bool EXACT, bool CLAMP, bool WT, bool LDSRC = false>
 __device__ __forceinline__ void window_pass(const PMConst &c, const SrcView (&SV)[G],
                                             const Proj (&P)[G], const float (&iz0)[G],
                                             const float (&sc)[G], const float *s_ref, int lbase,
@@ -299,7 +308,7 @@ __device__ __forceinline__ void window_pass(const PMConst &c, const SrcView (&SV
         __builtin_amdgcn_s_setprio(1);
 #pragma unroll
         for (int k = 0; k < G; ++k)
-            tap_issue<EXACT, CLAMP>(SV[k], P[k], giu[k], rx[k], ry[k], rz[k], fi, v[k], fx[k], fy[k]);
+            tap_issue<EXACT, CLAMP, LDSRC>(SV[k], P[k], giu[k], rx[k], ry[k], rz[k], fi, v[k], fx[k], fy[k]);
         __builtin_amdgcn_s_setprio(0);
         if (ii + 1 < nt) {
             ++ii;
@@ -346,10 +355,15 @@ __device__ __forceinline__ void window_pass(const PMConst &c, const SrcView (&SV
 // FP32 moment sums well conditioned when the two views differ in brightness).
 // SS[k] is the source of hypothesis k: all the same source (several planes
 // against one source) or G sources of one plane (view selection).
-template <int G>
+// With LDSRC the window taps read the LDS box `staged` describes (its quad
+// is the LDS array; see tap_issue) instead of the source in global memory;
+// the centre texel still comes from global memory.  Same arithmetic, same
+// result.
+template <int G, bool LDSRC = false>
 __device__ __forceinline__ void ncc_cost_group(const PMConst &c, const SrcLds *const (&SS)[G],
                                                const Proj (&P)[G], const float *s_ref, int lbase,
-                                               const RefStats &R, float (&cost)[G])
+                                               const RefStats &R, float (&cost)[G],
+                                               const SrcView *staged = nullptr)
 {
     bool valid[G];
     float iz0[G], sc[G], ss[G], sss[G], srs[G];
@@ -374,7 +388,16 @@ __device__ __forceinline__ void ncc_cost_group(const PMConst &c, const SrcLds *c
         const float my = fr * (fabsf(jyx) + fabsf(jyy)) * (1.f + 2.f * delta) + 1.f;
         edge = edge || !(cx - mx >= S.lox && cx + mx <= S.hix && cy - my >= S.loy && cy + my <= S.hiy);
     }
-    if (__ballot(far) != 0ull) {
+    if constexpr (LDSRC) {
+        static_assert(G == 1, "one staged source per pass");
+        const SrcView LV[1] = {*staged};
+        if (__ballot(far) != 0ull)
+            window_pass<1, true, true, false, true>(c, LV, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
+        else if (__ballot(edge) != 0ull)
+            window_pass<1, false, true, false, true>(c, LV, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
+        else
+            window_pass<1, false, false, false, true>(c, LV, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
+    } else if (__ballot(far) != 0ull) {
         window_pass<G, true, true, false>(c, SV, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
     } else if (R.wt != nullptr) {
         if (__ballot(edge) != 0ull)
@@ -990,6 +1013,144 @@ __global__ __launch_bounds__(kThreads) void k_eval_cost(const ViewDesc *__restri
     }
 }
 
+// ---------------------------------------------------------------------------
+// LDS staging of source footprints (measurement kernel).  Cost of the stored
+// hypotheses against every source, sources outer: per (tile, source) the
+// workgroup reduces the bounding box of every window's projection (the
+// projective image of a square window is the convex hull of its corners'
+// images; +1 texel for the bilinear neighbour, clamped to the image), and
+// when the box has at most kStageCap texels it copies that box of the quad
+// texture into LDS with row-contiguous loads, after which every tap reads
+// LDS instead of the L1/L2-cached global texture.  STAGE = false runs the
+// same loops with global gathers only, so the two variants time the gather
+// path alone; their outputs are identical (same texels, same arithmetic).
+// ---------------------------------------------------------------------------
+constexpr int kStageCap = 4096;      // texels: 16 KB of LDS
+
+__device__ __forceinline__ int wave_min(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
+template <int MAXS, bool STAGE>
+__global__ __launch_bounds__(kThreads) void k_eval_stage(const ViewDesc *__restrict__ views,
+                                                         const Tile *__restrict__ tiles, PMConst c,
+                                                         float *out, unsigned int *n_staged)
+{
+    __shared__ float s_ref[kLdsH * kLdsW];
+    __shared__ SrcLds s_src[MAXS];
+    __shared__ uint32_t s_stage[STAGE ? kStageCap : 1];
+    __shared__ int s_box[4];
+    const Tile t = tiles[blockIdx.x];
+    const ViewDesc &V = views[t.view];
+    tile_prologue<MAXS, kTileW>(V, t, s_ref, s_src);
+    __syncthreads();
+    const size_t HW = (size_t)V.W * V.H;
+    constexpr int kPix = kTileW * kTileH / kThreads;          // pixels per thread
+    bool inb[kPix];
+    int pix[kPix], lb[kPix];
+    float du[kPix], dv[kPix], hd[kPix], ha[kPix], hb[kPix];
+    RefStats R[kPix];
+#pragma unroll
+    for (int j = 0; j < kPix; ++j) {
+        const int k = threadIdx.x + j * kThreads;
+        const int tx = k & (kTileW - 1), ty = k / kTileW;
+        const int x = t.x0 + tx, y = t.y0 + ty;
+        inb[j] = x < V.W && y < V.H;
+        pix[j] = inb[j] ? y * V.W + x : 0;
+        lb[j] = (ty + kMaxRadius) * kLdsW + tx + kMaxRadius;
+        du[j] = (float)(tx - kTileW / 2);
+        dv[j] = (float)(ty - kTileH / 2);
+        R[j] = ref_stats(c, s_ref, lb[j], kLdsW);
+        const float4 h = V.hyp[pix[j]];
+        hd[j] = h.x;
+        ha[j] = -h.y / h.w;
+        hb[j] = -h.z / h.w;
+    }
+    const float fr = (float)c.radius;
+    for (int s = 0; s < V.n_src; ++s) {
+        const SrcLds &Ss = s_src[s];
+        bool staged = false;
+        SrcView lv;
+        if constexpr (STAGE) {
+            // this thread's windows: bounding box in anchor-relative texels
+            int x0 = INT_MAX, y0 = INT_MAX, x1 = INT_MIN, y1 = INT_MIN;
+            bool unbounded = false;
+#pragma unroll
+            for (int j = 0; j < kPix; ++j) {
+                if (!inb[j]) continue;
+                const Proj P = make_proj(Ss, du[j], dv[j], hd[j], ha[j], hb[j]);
+#pragma unroll
+                for (int cc = 0; cc < 4; ++cc) {
+                    const float ox = (cc & 1) ? fr : -fr, oy = (cc & 2) ? fr : -fr;
+                    const float qx = fmaf(oy, P.hvx, fmaf(ox, P.hux, P.q0x));
+                    const float qy = fmaf(oy, P.hvy, fmaf(ox, P.huy, P.q0y));
+                    const float qz = fmaf(oy, P.hvz, fmaf(ox, P.huz, P.q0z));
+                    if (!(qz > 0.f)) { unbounded = true; continue; }
+                    const float px = __builtin_amdgcn_fmed3f(qx / qz, Ss.lox, Ss.hix);
+                    const float py = __builtin_amdgcn_fmed3f(qy / qz, Ss.loy, Ss.hiy);
+                    x0 = min(x0, (int)floorf(px) - 1);
+                    y0 = min(y0, (int)floorf(py) - 1);
+                    x1 = max(x1, (int)floorf(px) + 2);
+                    y1 = max(y1, (int)floorf(py) + 2);
+                }
+            }
+            if (unbounded) { x0 = INT_MIN / 2; y0 = INT_MIN / 2; x1 = INT_MAX / 2; y1 = INT_MAX / 2; }
+            if (threadIdx.x == 0) { s_box[0] = INT_MAX; s_box[1] = INT_MAX; s_box[2] = INT_MIN; s_box[3] = INT_MIN; }
+            __syncthreads();
+            x0 = wave_min(x0); y0 = wave_min(y0); x1 = wave_max(x1); y1 = wave_max(y1);
+            if ((threadIdx.x & 63) == 0) {
+                atomicMin(&s_box[0], x0); atomicMin(&s_box[1], y0);
+                atomicMax(&s_box[2], x1); atomicMax(&s_box[3], y1);
+            }
+            __syncthreads();
+            // absolute texel box, clamped to the image (clamped taps stay inside it)
+            const int bx0 = max(s_box[0] + Ss.ax, 0), by0 = max(s_box[1] + Ss.ay, 0);
+            const int bx1 = min(s_box[2] + Ss.ax, Ss.W - 1), by1 = min(s_box[3] + Ss.ay, Ss.H - 1);
+            const int bw = bx1 - bx0 + 1, bh = by1 - by0 + 1;
+            staged = bw > 0 && bh > 0 && (long long)bw * bh <= kStageCap;
+            if (staged) {
+                // row-contiguous copy of the box
+                int xx = threadIdx.x % bw, yy = threadIdx.x / bw;
+                const uint32_t *src = Ss.quad;
+                for (int i = threadIdx.x; i < bw * bh; i += kThreads) {
+                    s_stage[i] = src[(size_t)(by0 + yy) * Ss.W + bx0 + xx];
+                    xx += kThreads % bw;
+                    yy += kThreads / bw;
+                    if (xx >= bw) { xx -= bw; ++yy; }
+                }
+                lv = view_of(Ss);
+                lv.W4 = bw;
+                lv.base4 = (Ss.ay - by0) * bw + (Ss.ax - bx0);
+                lv.quad = s_stage;
+                if (threadIdx.x == 0 && n_staged) atomicAdd(n_staged, 1u);
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int j = 0; j < kPix; ++j) {
+            const Proj P = make_proj(Ss, du[j], dv[j], hd[j], ha[j], hb[j]);
+            const Proj PP[1] = {P};
+            const SrcLds *const SS[1] = {&Ss};
+            float cost[1];
+            if (STAGE && staged)
+                ncc_cost_group<1, true>(c, SS, PP, s_ref, lb[j], R[j], cost, &lv);
+            else
+                ncc_cost_group<1>(c, SS, PP, s_ref, lb[j], R[j], cost);
+            if (inb[j]) out[(size_t)s * HW + pix[j]] = total_cost(c, Ss, P, cost[0], V.pass, du[j], dv[j]);
+        }
+        if constexpr (STAGE) __syncthreads();     // the next source's copy overwrites the box
+    }
+}
+
 // Filter rule of the plane (d, a, b) against the G sources s0 .. s0+G-1,
 // their costs from one shared window pass: a source is consistent when
 // 1 - cost >= min NCC, its triangulation angle passes and, in the geometric
@@ -1242,6 +1403,22 @@ hipError_t launch_eval_cost(int maxs, const ViewDesc *views, const Tile *tiles, 
     if (maxs <= 8) k_eval_cost<8><<<ntiles, kThreads, 0, st>>>(views, tiles, c, out);
     else if (maxs <= 16) k_eval_cost<16><<<ntiles, kThreads, 0, st>>>(views, tiles, c, out);
     else k_eval_cost<32><<<ntiles, kThreads, 0, st>>>(views, tiles, c, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_eval_stage(int maxs, int stage, const ViewDesc *views, const Tile *tiles, int ntiles,
+                             const PMConst &c, float *out, unsigned int *n_staged, hipStream_t st)
+{
+    if (ntiles <= 0) return hipSuccess;
+    if (stage) {
+        if (maxs <= 8) k_eval_stage<8, true><<<ntiles, kThreads, 0, st>>>(views, tiles, c, out, n_staged);
+        else if (maxs <= 16) k_eval_stage<16, true><<<ntiles, kThreads, 0, st>>>(views, tiles, c, out, n_staged);
+        else k_eval_stage<32, true><<<ntiles, kThreads, 0, st>>>(views, tiles, c, out, n_staged);
+    } else {
+        if (maxs <= 8) k_eval_stage<8, false><<<ntiles, kThreads, 0, st>>>(views, tiles, c, out, n_staged);
+        else if (maxs <= 16) k_eval_stage<16, false><<<ntiles, kThreads, 0, st>>>(views, tiles, c, out, n_staged);
+        else k_eval_stage<32, false><<<ntiles, kThreads, 0, st>>>(views, tiles, c, out, n_staged);
+    }
     return hipGetLastError();
 }
 

@@ -201,6 +201,19 @@ class MvsContext:
                                            fptr(out)))
         return out
 
+    def eval_cost_staging(self, ref, srcs, depth_range, depth, normal_uvd, stage, repeats=5, pass_=0):
+        """vsm_eval_cost_staging: (costs, kernel ms per launch, staged fraction)."""
+        r, keep = make_req(ref, srcs, depth_range)
+        h, w = self.sizes[int(ref)]
+        d = np.ascontiguousarray(depth, dtype=np.float32).reshape(h, w)
+        nn = np.ascontiguousarray(normal_uvd, dtype=np.float32).reshape(h, w, 3)
+        out = np.zeros((len(srcs), h, w), np.float32)
+        ms, frac = ctypes.c_float(), ctypes.c_float()
+        self._check(self.lib.vsm_eval_cost_staging(self.h, ctypes.byref(r), int(pass_), fptr(d), fptr(nn),
+                                                   int(stage), int(repeats), fptr(out), ctypes.byref(ms),
+                                                   ctypes.byref(frac)))
+        return out, ms.value, frac.value
+
     def debug_init(self, ref, srcs, depth_range, pass_=0):
         r, keep = make_req(ref, srcs, depth_range)
         h, w = self.sizes[int(ref)]

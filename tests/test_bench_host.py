@@ -96,3 +96,17 @@ def test_warmup_discards_open_batches_and_photometric_only():
         r.step(k)
     assert r.S == 10 and ctx.log[-1] == ("fuse", (1, 2), 0, 1)
     assert not any(e[0] == "begin" and e[2] == 1 for e in ctx.log)
+
+
+def test_c2_oracle_fixture_and_bar():
+    """The committed oracle fixture for the 10 x 1024^2 workload exists, was
+    made by the FP64 oracle over the full pipeline at stride 1, and the
+    benchmark's accuracy bar is its error + 0.05 m."""
+    bar = bench.oracle_bar("c2")
+    assert bar is not None
+    assert "stride 1" in bar["workload"] and "5+5" in bar["workload"]
+    assert "pm_oracle.c" in bar["solver"]
+    assert 0.0 < bar["dsm_median_abs_err_m"] < 0.1
+    assert bar["dsm_coverage"] > 0.9 and bar["kept_fraction"] > 0.5
+    assert bench.ORACLE_BAR_M == 0.05
+    assert bench.oracle_bar("c9") is None

@@ -565,7 +565,9 @@ def bench_forms(a, cfg, steps, warmup, with_cpu=True):
                                         f"window step {step}, {tile_w} x {512 // tile_w} tiles "
                                         f"(autotuner)"},
                    "autotune": tune_rep,
-                   "source_draws": "per 32x4-pixel group" if a.draw_group else "per pixel"},
+                   "source_draws": ("per pixel group of one wave (tile width x 128 / tile width: "
+                                    f"plain 32 x 4, batched {tile_w} x {128 // tile_w})"
+                                    if a.draw_group else "per pixel")},
         "plain_views_per_s": round(p["views_per_s"], 4),
         "batched_over_plain": round(b["views_per_s"] / p["views_per_s"], 3),
         "batched_over_plain_same_step": same,
@@ -655,8 +657,10 @@ def bench_serving(a):
     warm.drain()
     loop = ServeLoop(ctx, on_complete=on_complete, geometric=True)
     ctx.reset_stats()
+    ctx.set_timing(True)
     served = loop.run_schedule(arrivals, reqs)
     st = ctx.stats()
+    ctx.set_timing(False)
     served_err = site_errors()
     log("served", {k: v for k, v in served.items() if k not in ("tickets", "latency_s")},
         "dsm", served_err, fused)
@@ -705,6 +709,7 @@ def bench_serving(a):
                      "served_p50_ms": round(burst["p50_ms"], 1),
                      "served_p99_ms": round(burst["p99_ms"], 1)},
         "serve_steps": served["steps"],
+        "served_gpu_halfstep_s": round(st["halfstep_ms"] / 1e3, 3),
         "hbm_peak_gb": round(st["hbm_bytes_peak"] / 1e9, 2),
         "roofline": roofline(st, "c4"),
         "cpu_baseline": None,
@@ -784,8 +789,8 @@ def main():
     ap.add_argument("--no-nested", action="store_true", help="c5: skip the nested c2 workload")
     ap.add_argument("--cpu-crop", type=int, default=512)
     ap.add_argument("--draw-group", type=int, default=1, choices=(0, 1),
-                    help="Monte-Carlo source draws keyed by pixel (0) or by 32x4-pixel group (1, "
-                         "both forms)")
+                    help="Monte-Carlo source draws keyed by pixel (0) or by the pixel group one "
+                         "wave covers (1, tile width x 128 / tile width; both forms)")
     ap.add_argument("--forms", default="plain,batched", help="comma list of plain,batched")
     ap.add_argument("--build-only", action="store_true", help="build the synthetic site(s) and exit")
     ap.add_argument("--copy-calibration-mb", type=int, default=0)

@@ -74,6 +74,9 @@ struct ServeReq {
     // device state while active
     ViewState vs;
     int host_step = 0;
+    int slot = -1;                          // its descriptor in the served view array
+    Tile *dtiles = nullptr;                 // its half-step tiles (view index = slot)
+    int ntiles = 0, tile_w = 0;
     // results (device) when done: depth, normal (3), cost, consistency mask
     float *out_depth = nullptr, *out_normal = nullptr, *out_cost = nullptr;
     uint32_t *out_mask = nullptr;
@@ -119,8 +122,10 @@ struct vsm_ctx {
     std::vector<ServeReq *> active;
     std::map<uint64_t, std::unique_ptr<ServeReq>> reqs;
     ViewDesc *serve_views = nullptr;
-    Tile *serve_tiles = nullptr;
-    int serve_cap_tiles = 0;
+    std::vector<ServeReq *> serve_slots;   // request per descriptor slot (null: free)
+    Tile *serve_tiles = nullptr;           // the active requests' tiles, concatenated
+    int serve_cap_tiles = 0, serve_ntiles = 0;
+    bool serve_dirty = true;               // serve_tiles does not match the active list
     // staged batched solves
     std::map<int, std::unique_ptr<Batch>> batches;
     int next_batch = 1;
@@ -1743,6 +1748,42 @@ static void serve_drop_keep(vsm_ctx *c, ServeReq &r)
 
 // State, outputs and initial hypotheses of an admitted request; on failure
 // everything it allocated is released again.
+// The request's half-step tiles at the context's tile width on the device,
+// its slot as their view index.
+static int serve_build_tiles(vsm_ctx *c, ServeReq &r)
+{
+    const int tw = c->params.tile_width;
+    std::vector<Tile> tl;
+    tiles_for(r.vs.desc, r.slot, tl, tw);
+    Tile *dt = nullptr;
+    int rc = dalloc(c, &dt, tl.size());
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(dt, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        dfree(c, dt);
+        return fail(c, VSM_EDEVICE, "request tiles: %s", hipGetErrorString(e));
+    }
+    dfree(c, r.dtiles);
+    r.dtiles = dt;
+    r.ntiles = (int)tl.size();
+    r.tile_w = tw;
+    c->serve_dirty = true;
+    return VSM_OK;
+}
+
+static void serve_release_slot(vsm_ctx *c, ServeReq &r)
+{
+    if (r.slot >= 0) c->serve_slots[r.slot] = nullptr;
+    r.slot = -1;
+    dfree(c, r.dtiles);
+    r.dtiles = nullptr;
+    r.ntiles = 0;
+    c->serve_dirty = true;
+}
+
+// State, outputs, descriptor slot, tiles and initial hypotheses of an
+// admitted request; on failure everything it took is released again.
 static int serve_start(vsm_ctx *c, ServeReq &r)
 {
     vsm_view_req q;
@@ -1756,33 +1797,33 @@ static int serve_start(vsm_ctx *c, ServeReq &r)
     if (rc) return rc;
     const ImageRec &img = c->images.at(r.ref);
     const size_t hw = (size_t)r.vs.desc.W * r.vs.desc.H;
-    std::vector<Tile> tl;
-    tiles_for(r.vs.desc, 0, tl);
-    ViewDesc *dv = nullptr;
-    Tile *dt = nullptr;
-    rc = dalloc(c, &r.out_depth, hw * 6);
+    for (int k = 0; k < (int)c->serve_slots.size() && r.slot < 0; ++k)
+        if (!c->serve_slots[k]) r.slot = k;
+    if (r.slot < 0) rc = fail(c, VSM_EINVAL, "no free serving slot");
+    if (!rc) {
+        c->serve_slots[r.slot] = &r;
+        rc = serve_build_tiles(c, r);
+    }
+    if (!rc) rc = dalloc(c, &r.out_depth, hw * 6);
     if (!rc && !img.depth[r.pass]) rc = dalloc(c, &r.keep_depth, hw);
     if (!rc && !img.normal[r.pass]) rc = dalloc(c, &r.keep_normal, hw * 3);
-    if (!rc) rc = dalloc(c, &dv, 1);
-    if (!rc) rc = dalloc(c, &dt, tl.size());
     if (!rc) {
         const PMConst pc = make_const(c->params);
+        ViewDesc *dv = c->serve_views + r.slot;
+        r.vs.desc.step = 0;
         hipError_t e = hipMemcpyAsync(dv, &r.vs.desc, sizeof(ViewDesc), hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(dt, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream);
         if (e == hipSuccess && r.pass == 1)     // start from the photometric depth and normals
             e = launch_load_hyp(dv, 0, (int)hw, img.depth[0], img.normal[0], pc.cos_max_normal, c->stream);
         if (e == hipSuccess)
-            e = launch_init(dv, dt, (int)tl.size(), pc, r.pass == 0 ? 1 : 0, kTileW, c->stream);
+            e = launch_init(c->serve_views, r.dtiles, r.ntiles, pc, r.pass == 0 ? 1 : 0, r.tile_w, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) rc = fail(c, VSM_EDEVICE, "request start failed: %s", hipGetErrorString(e));
     }
-    dfree(c, dv);
-    dfree(c, dt);
     if (rc) {
         dfree(c, r.out_depth);
         r.out_depth = nullptr;
         serve_drop_keep(c, r);
+        serve_release_slot(c, r);
         release_view(c, r.vs);
         return rc;
     }
@@ -1898,6 +1939,7 @@ static int serve_submit(vsm_ctx *c, const vsm_view_req *req, int pass, int filte
     // resident serving scratch is allocated before any admission decision
     int rc = VSM_OK;
     if (!c->serve_views && (rc = dalloc(c, &c->serve_views, kServeMaxViews))) return rc;
+    c->serve_slots.resize(kServeMaxViews, nullptr);
     if (!tile_width_supported(c->params.tile_width, req->n_src))
         return fail(c, VSM_EINVAL, "tile_width %d is not available for %d sources", c->params.tile_width,
                     req->n_src);
@@ -1937,32 +1979,43 @@ int vsm_serve_step(vsm_ctx *c, int32_t *n_active, int32_t *n_queued)
     if (rc) return rc;
     const int na = (int)c->active.size();
     if (na > 0) {
-        std::vector<ViewDesc> hv(na);
-        std::vector<Tile> tiles;
-        int maxs = 1;
-        for (int v = 0; v < na; ++v) {
-            ServeReq &r = *c->active[v];
-            hv[v] = r.vs.desc;
-            hv[v].step = r.host_step;
-            maxs = std::max(maxs, hv[v].n_src);
-            tiles_for(hv[v], v, tiles, c->params.tile_width);
-        }
-        if ((int)tiles.size() > c->serve_cap_tiles) {
-            dfree(c, c->serve_tiles);
-            c->serve_tiles = nullptr;
-            c->serve_cap_tiles = 0;
-            if ((rc = dalloc(c, &c->serve_tiles, tiles.size() * 2))) return rc;
-            c->serve_cap_tiles = (int)tiles.size() * 2;
+        // The descriptors and tiles stay on the device while a request is
+        // active: a step re-concatenates the tile lists only when the active
+        // list (or the tile width) changed, and the device step counters
+        // advance after each half-step (k_advance), in step with host_step.
+        int maxs = 1, nslots = 0;
+        for (ServeReq *r : c->active) {
+            maxs = std::max(maxs, r->vs.desc.n_src);
+            nslots = std::max(nslots, r->slot + 1);
+            if (r->tile_w != c->params.tile_width && (rc = serve_build_tiles(c, *r))) return rc;
         }
         if (!tile_width_supported(c->params.tile_width, maxs))
             return fail(c, VSM_EINVAL, "tile_width %d is not available for %d sources", c->params.tile_width,
                         maxs);
+        if (c->serve_dirty) {
+            long long total = 0;
+            for (ServeReq *r : c->active) total += r->ntiles;
+            if (total > c->serve_cap_tiles) {
+                dfree(c, c->serve_tiles);
+                c->serve_tiles = nullptr;
+                c->serve_cap_tiles = 0;
+                if ((rc = dalloc(c, &c->serve_tiles, (size_t)total * 2))) return rc;
+                c->serve_cap_tiles = (int)(total * 2);
+            }
+            long long off = 0;
+            for (ServeReq *r : c->active) {
+                HIPCHK(c, hipMemcpyAsync(c->serve_tiles + off, r->dtiles, r->ntiles * sizeof(Tile),
+                                         hipMemcpyDeviceToDevice, c->stream));
+                off += r->ntiles;
+            }
+            c->serve_ntiles = (int)total;
+            c->serve_dirty = false;
+        }
         const PMConst pc = make_const(c->params);
-        HIPCHK(c, hipMemcpyAsync(c->serve_views, hv.data(), na * sizeof(ViewDesc), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->serve_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream));
         if ((rc = record_halfstep(c, maxs, c->params.tile_width, c->serve_views, c->serve_tiles,
-                                  (int)tiles.size(), pc, c->timing)))
+                                  c->serve_ntiles, pc, c->timing)))
             return rc;
+        HIPCHK(c, launch_advance(c->serve_views, nslots, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         // Every active request has advanced one half-step.  The finished ones
         // leave the active list before any fallible work on them, so a
@@ -1973,13 +2026,15 @@ int vsm_serve_step(vsm_ctx *c, int32_t *n_active, int32_t *n_queued)
             ServeReq &r = *c->active[v];
             r.host_step += 1;
             if (r.host_step < r.vs.desc.num_steps) still.push_back(&r);
-            else retire.emplace_back(&r, v);
+            else retire.emplace_back(&r, r.slot);
         }
+        if (!retire.empty()) c->serve_dirty = true;
         c->active.swap(still);
         int dev_err = VSM_OK;
         for (auto &rv : retire) {
             ServeReq &r = *rv.first;
             const int e = serve_finish(c, r, c->serve_views + rv.second);
+            serve_release_slot(c, r);
             release_view(c, r.vs);
             serve_drop_keep(c, r);
             r.done = true;

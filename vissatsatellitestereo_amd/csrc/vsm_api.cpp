@@ -1355,7 +1355,7 @@ int vsm_fuse_dsm(vsm_ctx *c, const vsm_dsm_grid *grid, const int32_t *ids, int32
     unsigned long long *cells = nullptr;
     float *stack = nullptr, *tmp = nullptr, *fused = nullptr;
     double *mats = nullptr;
-    rc = dalloc(c, &cells, ncell);
+    rc = dalloc(c, &cells, 2 * ncell);        // two key grids, used by alternate views
     if (!rc) rc = dalloc(c, &stack, ncell * n);
     if (!rc) rc = dalloc(c, &tmp, ncell);
     if (!rc) rc = dalloc(c, &fused, ncell);
@@ -1368,13 +1368,15 @@ int vsm_fuse_dsm(vsm_ctx *c, const vsm_dsm_grid *grid, const int32_t *ids, int32
     HIPCHK(c, hipMemcpyAsync(mats, hm.data(), hm.size() * 8, hipMemcpyHostToDevice, c->stream));
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     int64_t pixels = 0;
+    // view v rasterises into key grid v % 2; its per-view DSM pass zeroes
+    // the other grid for view v + 1
+    HIPCHK(c, launch_fill_u64n(cells, ncell, 0ull, c->stream));
     for (int v = 0; v < n; ++v) {
         const ImageRec &r = c->images[ids[v]];
-        HIPCHK(c, launch_fill_u64n(cells, ncell, 0ull, c->stream));
+        unsigned long long *kv = cells + (size_t)(v & 1) * ncell, *knext = cells + (size_t)((v + 1) & 1) * ncell;
         HIPCHK(c, launch_rasterize_view(r.depth[depth_kind], r.W, r.H, mats + 16 * v, g,
-                                        c->has_geo ? &c->geo : nullptr, cells, c->stream));
-        HIPCHK(c, launch_decode_fill_f32(cells, g.e_size, g.n_size, tmp, c->stream));
-        HIPCHK(c, launch_median3(tmp, g.e_size, g.n_size, stack + (size_t)v * ncell, c->stream));
+                                        c->has_geo ? &c->geo : nullptr, kv, c->stream));
+        HIPCHK(c, launch_view_dsm(kv, knext, g.e_size, g.n_size, stack + (size_t)v * ncell, c->stream));
         pixels += (int64_t)r.W * r.H;
     }
     HIPCHK(c, launch_fuse_stack(stack, n, ncell, min_measurements, tmp, c->stream));

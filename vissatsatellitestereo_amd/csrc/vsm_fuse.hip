@@ -305,6 +305,87 @@ __global__ void k_median3(const float *__restrict__ in, int e_size, int n_size,
     out[i] = median_sorted<float, 9>(v, n);
 }
 
+// Per-view DSM in one pass over a tile of kVT_W x kVT_H cells: decode the
+// cell keys, fill each empty cell with the median of its non-empty 3x3
+// neighbours (the unfilled keys, as k_decode_fill), then the 3x3 median of
+// the filled values (replicated border, NaN excluded, as k_median3) into
+// `out`.  The keys of the tile and a 2-cell halo are staged in LDS, the
+// filled values of the tile and a 1-cell halo (at border-clamped positions)
+// too, so each key is read once per tile instead of by three kernels.  The
+// tile's cells of `clear` (the other key grid, which the next view
+// rasterises into) are zeroed; `keys` and `clear` must differ, since other
+// tiles read this tile's keys as their halo.
+constexpr int kVT_W = 64, kVT_H = 16;
+
+__global__ void __launch_bounds__(256) k_view_dsm(const unsigned long long *__restrict__ keys,
+                                                  unsigned long long *__restrict__ clear, int e_size,
+                                                  int n_size, float *__restrict__ out)
+{
+    constexpr int KW = kVT_W + 4, KH = kVT_H + 4, FW = kVT_W + 2, FH = kVT_H + 2;
+    __shared__ unsigned long long sk[KH * KW];
+    __shared__ float sf[FH * FW];
+    const int c0 = blockIdx.x * kVT_W, r0 = blockIdx.y * kVT_H;
+    for (int t = threadIdx.x; t < KH * KW; t += blockDim.x) {
+        const int ky = t / KW, kx = t - ky * KW;
+        const int r = r0 - 2 + ky, c = c0 - 2 + kx;
+        sk[t] = (r >= 0 && c >= 0 && r < n_size && c < e_size) ? keys[(size_t)r * e_size + c] : 0ull;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < FH * FW; t += blockDim.x) {
+        const int fy = t / FW, fx = t - fy * FW;
+        // the filled value at the border-clamped position (the median below
+        // reads replicated borders)
+        const int rc = min(max(r0 - 1 + fy, 0), n_size - 1), cc = min(max(c0 - 1 + fx, 0), e_size - 1);
+        const int ky = rc - (r0 - 2), kx = cc - (c0 - 2);
+        const unsigned long long k = sk[ky * KW + kx];
+        float f;
+        if (k != 0ull) {
+            f = (float)dec_f64(k);
+        } else {
+            double v[9];
+            int n = 0;
+#pragma unroll
+            for (int dr = -1; dr <= 1; ++dr)
+#pragma unroll
+                for (int dc = -1; dc <= 1; ++dc) {
+                    const unsigned long long kk = sk[(ky + dr) * KW + kx + dc];
+                    const int slot = (dr + 1) * 3 + (dc + 1);
+                    v[slot] = kk ? dec_f64(kk) : (double)INFINITY;
+                    n += kk ? 1 : 0;
+                }
+            if (n == 0) {
+                f = NAN;
+            } else {
+                sort_small<double, 9>(v, 9);
+                f = (float)median_sorted<double, 9>(v, n);
+            }
+        }
+        sf[t] = f;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < kVT_W * kVT_H; t += blockDim.x) {
+        const int ty = t / kVT_W, tx = t - ty * kVT_W;
+        const int r = r0 + ty, c = c0 + tx;
+        if (r >= n_size || c >= e_size) continue;
+        float v[9];
+        int n = 0;
+#pragma unroll
+        for (int dr = -1; dr <= 1; ++dr)
+#pragma unroll
+            for (int dc = -1; dc <= 1; ++dc) {
+                const float x = sf[(ty + 1 + dr) * FW + tx + 1 + dc];
+                const int slot = (dr + 1) * 3 + (dc + 1);
+                v[slot] = x == x ? x : INFINITY;
+                n += x == x ? 1 : 0;
+            }
+        const size_t i = (size_t)r * e_size + c;
+        clear[i] = 0ull;
+        if (n == 0) { out[i] = NAN; continue; }
+        sort_small<float, 9>(v, 9);
+        out[i] = median_sorted<float, 9>(v, n);
+    }
+}
+
 // Bitonic sort network over N registers (static indices only).
 template <int N>
 __device__ __forceinline__ void bitonic(float (&v)[N])
@@ -317,11 +398,11 @@ __device__ __forceinline__ void bitonic(float (&v)[N])
             for (int i = 0; i < N; ++i) {
                 const int l = i ^ j;
                 if (l > i) {
-                    const bool up = (i & k) == 0;
+                    // values are finite or +inf (no NaN): min/max order them
                     const float a = v[i], b = v[l];
-                    const bool sw = up ? (a > b) : (a < b);
-                    v[i] = sw ? b : a;
-                    v[l] = sw ? a : b;
+                    const float lo = fminf(a, b), hi = fmaxf(a, b);
+                    v[i] = (i & k) == 0 ? lo : hi;
+                    v[l] = (i & k) == 0 ? hi : lo;
                 }
             }
 }
@@ -438,6 +519,15 @@ hipError_t launch_decode_fill_f64(const unsigned long long *cell_max, int e_size
 {
     k_decode_fill<double><<<nblk((long long)e_size * n_size, 256), 256, 0, st>>>(cell_max, e_size,
                                                                               n_size, dsm);
+    return hipGetLastError();
+}
+
+hipError_t launch_view_dsm(const unsigned long long *keys, unsigned long long *clear, int e_size, int n_size,
+                           float *out, hipStream_t st)
+{
+    if (keys == clear) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((e_size + kVT_W - 1) / kVT_W), (unsigned)((n_size + kVT_H - 1) / kVT_H));
+    k_view_dsm<<<grid, 256, 0, st>>>(keys, clear, e_size, n_size, out);
     return hipGetLastError();
 }
 

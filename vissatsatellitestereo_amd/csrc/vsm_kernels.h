@@ -81,6 +81,10 @@ hipError_t launch_decode_fill_f32(const unsigned long long *cell_max, int e_size
 hipError_t launch_decode_fill_f64(const unsigned long long *cell_max, int e_size, int n_size,
                                   double *dsm, hipStream_t st);
 hipError_t launch_median3(const float *in, int e_size, int n_size, float *out, hipStream_t st);
+// Per-view DSM: decode + hole fill + 3x3 median of `keys` into `out` in one
+// pass; zeroes the same cells of `clear` (a different key grid).
+hipError_t launch_view_dsm(const unsigned long long *keys, unsigned long long *clear, int e_size, int n_size,
+                           float *out, hipStream_t st);
 hipError_t launch_fuse_stack(const float *stack, int nviews, long long ncell,
                              int min_measurements, float *out, hipStream_t st);
 hipError_t launch_fill_u32(unsigned int *p, long long n, unsigned int v, hipStream_t st);

@@ -98,6 +98,41 @@ def test_fuse_dsm_bit_exact(ctx, tiny):
     assert err < 0.3 and cov > 0.5
 
 
+@pytest.mark.parametrize("shape", [(1, 1), (1, 7), (9, 1), (17, 65), (33, 130)])
+def test_fuse_dsm_odd_grids_bit_exact(ctx, tiny, shape):
+    """Per-view DSMs on grids that are one cell thin or not a multiple of the
+    per-view kernel's 64 x 16-cell tile (single rows and columns take the
+    replicated border on both sides), against the oracle; the grids sit over
+    the middle of the site at 0.37 m cells, so cells are filled, empty and
+    hole-filled."""
+    n_size, e_size = shape
+    aoi = json.load(open(os.path.join(tiny["wd"], "aoi.json")))
+    ul_e = aoi["ul_easting"] + 0.4 * aoi["width"]
+    ul_n = aoi["ul_northing"] - 0.4 * aoi["height"]
+    grid = (ul_e, ul_n, 0.37, 0.37, e_size, n_size)
+    names = tiny["names"]
+    rng = np.random.default_rng(5)
+    ids, invs, depths = [], [], []
+    for i, n in enumerate(names):
+        d = sites.gt_depth(tiny, n).astype(np.float32)
+        d[rng.random(d.shape) < 0.3] = 0.0
+        h, w = d.shape
+        ctx.register_geometry(200 + i, w, h, tiny["P4"][n])
+        ctx.set_depth_map(200 + i, d, kind=1)
+        ids.append(200 + i)
+        invs.append(np.linalg.inv(tiny["P4"][n]))
+        depths.append(d)
+    fused, pv = ctx.fuse_dsm(grid, ids, depth_kind=1, min_measurements=2, inv_proj_mats=np.stack(invs),
+                             per_view=True)
+    views = [fo.view_dsm(d, inv, grid) for d, inv in zip(depths, invs)]
+    assert fused.shape == shape
+    for k in range(len(ids)):
+        assert _same(pv[k], views[k]), k
+    assert _same(fused, fo.fuse(np.stack(views), 2))
+    if n_size * e_size > 1:                     # the single cell lies in a gap of every view
+        assert any(np.isfinite(v).any() for v in views)
+
+
 def test_fuse_dsm_rejects_bad_arguments(ctx):
     from vissatsatellitestereo_amd._lib import VsmError
     with pytest.raises(VsmError):

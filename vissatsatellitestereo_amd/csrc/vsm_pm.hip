@@ -299,10 +299,17 @@ __device__ __forceinline__ void window_pass(const PMConst &c, const SrcView (&SV
     // tok: values the gathers must wait for (empty asm statements tie
     // them to the tap position, so the compiler keeps the issue after
     // the accumulation of the set being reloaded)
-    auto issue = [&](uint32_t (&v)[G], float (&fx)[G], float (&fy)[G], const float (&tok)[G]) {
+    // The tap's reference value and bilateral weight are read from LDS with
+    // its gathers, one accumulation ahead of their use.
+    auto issue = [&](uint32_t (&v)[G], float (&fx)[G], float (&fy)[G], const float (&tok)[G], float &rr,
+                     float &ww) {
         float fi = (float)(ii * st);
+        int la = lbase + (-rad + jj * st) * R.ldsw + (-rad + ii * st);
+        int wa = (jj * nt + ii) * kThreads;
 #pragma unroll
-        for (int k = 0; k < G; ++k) asm volatile("" : "+v"(fi) : "v"(tok[k]));
+        for (int k = 0; k < G; ++k) asm volatile("" : "+v"(fi), "+v"(la), "+v"(wa) : "v"(tok[k]));
+        rr = s_ref[la];
+        if constexpr (WT) ww = R.wt[wa];
         // a wave issuing gathers goes ahead of one accumulating, so its
         // loads leave early (-5 % per stride-1 half-step, neutral at stride 2)
         __builtin_amdgcn_s_setprio(1);
@@ -318,12 +325,11 @@ __device__ __forceinline__ void window_pass(const PMConst &c, const SrcView (&SV
             row_start<G, EXACT>(P, iz0, giu, giv, frad, -rad + jj * st, rx, ry, rz);
         }
     };
-    int ic = 0, jc = 0, tc = 0;               // next tap to accumulate
-    auto consume = [&](const uint32_t (&v)[G], const float (&fx)[G], const float (&fy)[G]) {
+    int ic = 0, jc = 0;                       // next tap to accumulate
+    auto consume = [&](const uint32_t (&v)[G], const float (&fx)[G], const float (&fy)[G], float rr, float ww) {
         const int dx = -rad + ic * st, dy = -rad + jc * st;
-        const float r = s_ref[lbase + dy * R.ldsw + dx] - R.rc;
-        const float w = WT ? R.wt[tc * kThreads] : tap_weight(c, r, dx, dy);
-        ++tc;
+        const float r = rr - R.rc;
+        const float w = WT ? ww : tap_weight(c, r, dx, dy);
         const float wr = w * r;
 #pragma unroll
         for (int k = 0; k < G; ++k) {
@@ -335,18 +341,19 @@ __device__ __forceinline__ void window_pass(const PMConst &c, const SrcView (&SV
         }
         if (++ic == nt) { ic = 0; ++jc; }
     };
-    issue(va, xa, ya, ss);
+    float ra = 0.f, wa = 0.f, rb = 0.f, wb = 0.f;
+    issue(va, xa, ya, ss, ra, wa);
     int t = 0;
 #pragma unroll 1
     for (; t + 1 < T; t += 2) {
-        issue(vb, xb, yb, ss);
-        consume(va, xa, ya);
+        issue(vb, xb, yb, ss, rb, wb);
+        consume(va, xa, ya, ra, wa);
         // set a's next gathers come after its last use (no register copy
         // at the loop edge, which would wait for them)
-        issue(va, xa, ya, ss);
-        consume(vb, xb, yb);
+        issue(va, xa, ya, ss, ra, wa);
+        consume(vb, xb, yb, rb, wb);
     }
-    if (t < T) consume(va, xa, ya);
+    if (t < T) consume(va, xa, ya, ra, wa);
 }
 
 // NCC cost of G plane hypotheses.  Each hypothesis' source samples are

@@ -36,6 +36,24 @@ constexpr int kGroupV = 4;      // sources per shared window pass in view select
 constexpr int kFarChunk = 8;    // propagation candidates loaded together per direction
 constexpr int kProbChunk = 4;   // sources whose neighbour probabilities are loaded together
 
+// Largest relative change of the homogeneous depth q.z across a window for
+// which the per-tap reciprocal is replaced by one Newton step from the exact
+// centre reciprocal: iz = iz0 (2 - qz iz0), relative error delta^2 <= 1e-6.
+// Satellite cameras (600 km range) give delta ~ 1e-5.
+constexpr float kDeltaMax = 1e-3f;
+
+// Depth range of the hypotheses a window pass may take without per-pixel
+// bounds work: half the view's depth range beyond either end (propagated
+// planes extrapolate past [depth_min, depth_max]).
+__device__ __forceinline__ float hyp_depth_lo(const ViewDesc &V)
+{
+    return V.depth_min - 0.5f * (V.depth_max - V.depth_min);
+}
+__device__ __forceinline__ float hyp_depth_hi(const ViewDesc &V)
+{
+    return V.depth_max + 0.5f * (V.depth_max - V.depth_min);
+}
+
 struct SrcLds {
     float A[12];                 // tile-centred ref -> src (anchor-relative), normalised
     float B[12];                 // src (anchor-relative) -> ref (tile-centre-relative)
@@ -45,6 +63,7 @@ struct SrcLds {
     const uint32_t *quad;
     const float4 *depth4;        // photometric depth of the source, 2x2 per texel
     float ptri, tri, g0, g1, g2;
+    int inner;                   // every window of the tile lies inside the source (see below)
 };
 
 struct Proj {
@@ -53,8 +72,50 @@ struct Proj {
     float hvx, hvy, hvz;         // d q / d dy
 };
 
+// Hypotheses a window pass may take without per-pixel bounds work: depth
+// within half the depth range outside [depth_min, depth_max], plane slopes
+// |a|, |b| at most those of the steepest normal the search generates.
+__device__ __forceinline__ float slope_bound(const PMConst &c)
+{
+    const float cm = c.cos_max_normal;
+    return cm > 0.f ? 1.01f * sqrtf(fmaxf(1.f - cm * cm, 0.f)) / cm : 0.f;
+}
+
+// 1 when every tap of every window of a tile (pixel box hu x hv around the
+// tile centre, plus the window radius) under every hypothesis with depth in
+// [dlo, dhi] and slopes |a|, |b| <= amax lands inside the source with a
+// 1.5-px guard, and the linearised per-tap reciprocal is accurate there
+// (delta <= kDeltaMax / 2).  The taps (u, v, 1, d + a dx + b dy) fill the box
+// |u| <= hu + r, |v| <= hv + r, d in [dlo - 2 amax r, dhi + 2 amax r]; with a
+// positive homogeneous depth at its 8 corners (it is affine there) their
+// projections lie in the convex hull of the corners' projections.
+__device__ int source_inner(const double (&A)[12], double hu, double hv, double r, double dlo,
+                            double dhi, double amax, double lox, double hix, double loy, double hiy)
+{
+    const double d0 = dlo - 2.0 * amax * r, d1 = dhi + 2.0 * amax * r;
+    double xmin = 1e300, xmax = -1e300, ymin = 1e300, ymax = -1e300, zmin = 1e300;
+    for (int k = 0; k < 8; ++k) {
+        const double u = (k & 1) ? hu + r : -(hu + r);
+        const double v = (k & 2) ? hv + r : -(hv + r);
+        const double d = (k & 4) ? d1 : d0;
+        const double qz = A[8] * u + A[9] * v + A[10] + A[11] * d;
+        if (!(qz > 0.0)) return 0;
+        const double px = (A[0] * u + A[1] * v + A[2] + A[3] * d) / qz;
+        const double py = (A[4] * u + A[5] * v + A[6] + A[7] * d) / qz;
+        xmin = fmin(xmin, px); xmax = fmax(xmax, px);
+        ymin = fmin(ymin, py); ymax = fmax(ymax, py);
+        zmin = fmin(zmin, qz);
+    }
+    const double g = 1.5;
+    if (!(xmin >= lox + g && xmax <= hix - g && ymin >= loy + g && ymax <= hiy - g)) return 0;
+    const double delta = r * (fabs(A[8]) + fabs(A[9]) + 2.0 * amax * fabs(A[11])) / zmin;
+    return delta <= 0.5 * (double)kDeltaMax ? 1 : 0;
+}
+
 // FP64 tile re-centring of one source's matrices (run by one thread per source).
-__device__ void setup_source(const ViewDesc &V, int s, int uc, int vc, SrcLds &o)
+// hu, hv: largest pixel offset from the tile centre.
+__device__ void setup_source(const ViewDesc &V, const PMConst &c, int s, int uc, int vc, int hu, int hv,
+                             SrcLds &o)
 {
     const double *M = V.M[s];
     const double *Mb = V.Mb[s];
@@ -95,6 +156,13 @@ __device__ void setup_source(const ViewDesc &V, int s, int uc, int vc, SrcLds &o
     o.depth4 = V.src_depth4[s];
     o.ptri = V.ptri[s]; o.tri = V.tri[s];
     o.g0 = V.g[s][0]; o.g1 = V.g[s][1]; o.g2 = V.g[s][2];
+    {
+        const double dr = (double)V.depth_max - (double)V.depth_min;
+        const double dlo = hyp_depth_lo(V) - 1e-3 * (fabs(dr) + 1.0);
+        const double dhi = hyp_depth_hi(V) + 1e-3 * (fabs(dr) + 1.0);
+        o.inner = source_inner(A, hu, hv, (double)c.radius, dlo, dhi, (double)slope_bound(c),
+                               (double)o.lox, (double)o.hix, (double)o.loy, (double)o.hiy);
+    }
 }
 
 __device__ __forceinline__ Proj make_proj(const SrcLds &S, float du, float dv, float d, float a,
@@ -244,12 +312,6 @@ __device__ __forceinline__ float blend_quad(uint32_t v, float fx, float fy)
     return fmaf(fy, bot - top, top);
 }
 
-// Largest relative change of the homogeneous depth q.z across a window for
-// which the per-tap reciprocal is replaced by one Newton step from the exact
-// centre reciprocal: iz = iz0 (2 - qz iz0), relative error delta^2 <= 1e-6.
-// Satellite cameras (600 km range) give delta ~ 1e-5.
-constexpr float kDeltaMax = 1e-3f;
-
 // Row start of row offset dy for each hypothesis: q (or, without EXACT, the
 // linearised 1/q.z) at column offset -r.
 template <int G, bool EXACT>
@@ -366,34 +428,53 @@ __device__ __forceinline__ void window_pass(const PMConst &c, const SrcView (&SV
 // is the LDS array; see tap_issue) instead of the source in global memory;
 // the centre texel still comes from global memory.  Same arithmetic, same
 // result.
+// lane_ok: this lane's hypotheses have depths in [hyp_depth_lo, hyp_depth_hi]
+// and slopes within slope_bound.  When every lane's are and every source is
+// `inner` for the tile, each window is known to lie inside its source with an
+// accurate linearised reciprocal, and the per-lane bounds work is skipped: the
+// pass then takes the variant the per-lane tests would (or the clamped one,
+// whose clamp changes no interior coordinate), so the costs are the same.
 template <int G, bool LDSRC = false>
 __device__ __forceinline__ void ncc_cost_group(const PMConst &c, const SrcLds *const (&SS)[G],
                                                const Proj (&P)[G], const float *s_ref, int lbase,
                                                const RefStats &R, float (&cost)[G],
-                                               const SrcView *staged = nullptr)
+                                               const SrcView *staged = nullptr, bool lane_ok = false)
 {
     bool valid[G];
     float iz0[G], sc[G], ss[G], sss[G], srs[G];
     SrcView SV[G];
     bool far = false, edge = false;
     const float fr = (float)c.radius;
+    bool inner = !LDSRC;
 #pragma unroll
-    for (int k = 0; k < G; ++k) {
-        SV[k] = view_of(*SS[k]);
-        const SrcView &S = SV[k];
-        valid[k] = P[k].q0z > 0.f;
-        iz0[k] = valid[k] ? __builtin_amdgcn_rcpf(P[k].q0z) : 1.f;
-        const float cx = P[k].q0x * iz0[k], cy = P[k].q0y * iz0[k];
-        valid[k] = valid[k] && cx >= S.lox && cx <= S.hix && cy >= S.loy && cy <= S.hiy;
-        sc[k] = valid[k] ? texel_at(S, cx, cy) : 0.f;
-        const float delta = fr * (fabsf(P[k].huz) + fabsf(P[k].hvz)) * fabsf(iz0[k]);
-        far = far || !(delta <= kDeltaMax);
-        // conservative half-extent of the projected window (+1 px guard)
-        const float jxx = (P[k].hux - cx * P[k].huz) * iz0[k], jxy = (P[k].hvx - cx * P[k].hvz) * iz0[k];
-        const float jyx = (P[k].huy - cy * P[k].huz) * iz0[k], jyy = (P[k].hvy - cy * P[k].hvz) * iz0[k];
-        const float mx = fr * (fabsf(jxx) + fabsf(jxy)) * (1.f + 2.f * delta) + 1.f;
-        const float my = fr * (fabsf(jyx) + fabsf(jyy)) * (1.f + 2.f * delta) + 1.f;
-        edge = edge || !(cx - mx >= S.lox && cx + mx <= S.hix && cy - my >= S.loy && cy + my <= S.hiy);
+    for (int k = 0; k < G; ++k) inner = inner && SS[k]->inner != 0;
+    if (inner && __ballot(!lane_ok) == 0ull) {
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            SV[k] = view_of(*SS[k]);
+            valid[k] = true;
+            iz0[k] = __builtin_amdgcn_rcpf(P[k].q0z);
+            sc[k] = texel_at(SV[k], P[k].q0x * iz0[k], P[k].q0y * iz0[k]);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            SV[k] = view_of(*SS[k]);
+            const SrcView &S = SV[k];
+            valid[k] = P[k].q0z > 0.f;
+            iz0[k] = valid[k] ? __builtin_amdgcn_rcpf(P[k].q0z) : 1.f;
+            const float cx = P[k].q0x * iz0[k], cy = P[k].q0y * iz0[k];
+            valid[k] = valid[k] && cx >= S.lox && cx <= S.hix && cy >= S.loy && cy <= S.hiy;
+            sc[k] = valid[k] ? texel_at(S, cx, cy) : 0.f;
+            const float delta = fr * (fabsf(P[k].huz) + fabsf(P[k].hvz)) * fabsf(iz0[k]);
+            far = far || !(delta <= kDeltaMax);
+            // conservative half-extent of the projected window (+1 px guard)
+            const float jxx = (P[k].hux - cx * P[k].huz) * iz0[k], jxy = (P[k].hvx - cx * P[k].hvz) * iz0[k];
+            const float jyx = (P[k].huy - cy * P[k].huz) * iz0[k], jyy = (P[k].hvy - cy * P[k].hvz) * iz0[k];
+            const float mx = fr * (fabsf(jxx) + fabsf(jxy)) * (1.f + 2.f * delta) + 1.f;
+            const float my = fr * (fabsf(jyx) + fabsf(jyy)) * (1.f + 2.f * delta) + 1.f;
+            edge = edge || !(cx - mx >= S.lox && cx + mx <= S.hix && cy - my >= S.loy && cy + my <= S.hiy);
+        }
     }
     if constexpr (LDSRC) {
         static_assert(G == 1, "one staged source per pass");
@@ -544,7 +625,8 @@ __device__ __forceinline__ T pick8(int k, T v0, T v1, T v2, T v3, T v4, T v5, T 
 template <int G>
 __device__ __forceinline__ void select_group(const PMConst &c, const SrcLds *s_src, int s0, float du,
                                              float dv, float d, float a, float b, const float *s_ref,
-                                             int lbase, const RefStats &R, float *s_c0, int tid)
+                                             int lbase, const RefStats &R, float *s_c0, int tid,
+                                             bool hyp_ok)
 {
     Proj P[G];
     float cg[G];
@@ -554,7 +636,7 @@ __device__ __forceinline__ void select_group(const PMConst &c, const SrcLds *s_s
         P[k] = make_proj(s_src[s0 + k], du, dv, d, a, b);
         SS[k] = &s_src[s0 + k];
     }
-    ncc_cost_group<G>(c, SS, P, s_ref, lbase, R, cg);
+    ncc_cost_group<G>(c, SS, P, s_ref, lbase, R, cg, nullptr, hyp_ok);
 #pragma unroll
     for (int k = 0; k < G; ++k) s_c0[(s0 + k) * kThreads + tid] = cg[k];
 }
@@ -566,7 +648,8 @@ __device__ __forceinline__ void eval_hyp_group(const PMConst &c, const SrcLds &S
                                                float du, float dv, const float (&hd)[kNHyp],
                                                const float (&ak)[kNHyp - 1],
                                                const float (&bk)[kNHyp - 1], const float *s_ref,
-                                               int lbase, const RefStats &R, float (&acc)[kNHyp - 1])
+                                               int lbase, const RefStats &R, float (&acc)[kNHyp - 1],
+                                               bool hyp_ok)
 {
     Proj P[G];
     float ph[G];
@@ -581,7 +664,7 @@ __device__ __forceinline__ void eval_hyp_group(const PMConst &c, const SrcLds &S
 #pragma unroll
     for (int k = 0; k < G; ++k)
         gt[k] = pass == 1 ? c.geom_reg * fminf(geo_error(c, Ss, P[k], du, dv), c.geom_max) : 0.f;
-    ncc_cost_group<G>(c, SS, P, s_ref, lbase, R, ph);
+    ncc_cost_group<G>(c, SS, P, s_ref, lbase, R, ph, nullptr, hyp_ok);
 #pragma unroll
     for (int k = 0; k < G; ++k) acc[K0 + k] += cnt * (pass == 1 ? ph[k] + gt[k] : ph[k]);
 }
@@ -592,12 +675,12 @@ __device__ __forceinline__ void eval_hyps(const PMConst &c, const SrcLds &Ss, fl
                                           float du, float dv, const float (&hd)[kNHyp],
                                           const float (&ak)[kNHyp - 1], const float (&bk)[kNHyp - 1],
                                           const float *s_ref, int lbase, const RefStats &R,
-                                          float (&acc)[kNHyp - 1])
+                                          float (&acc)[kNHyp - 1], bool hyp_ok)
 {
     if constexpr (K0 < kNHyp - 1) {
         constexpr int g = (kNHyp - 1 - K0) < G ? (kNHyp - 1 - K0) : G;
-        eval_hyp_group<g, K0>(c, Ss, cnt, pass, du, dv, hd, ak, bk, s_ref, lbase, R, acc);
-        eval_hyps<G, K0 + g>(c, Ss, cnt, pass, du, dv, hd, ak, bk, s_ref, lbase, R, acc);
+        eval_hyp_group<g, K0>(c, Ss, cnt, pass, du, dv, hd, ak, bk, s_ref, lbase, R, acc, hyp_ok);
+        eval_hyps<G, K0 + g>(c, Ss, cnt, pass, du, dv, hd, ak, bk, s_ref, lbase, R, acc, hyp_ok);
     }
 }
 
@@ -613,8 +696,8 @@ template <int TW> struct TileShape {
 // Common tile prologue: reference tile (replicated border) into LDS and the
 // tile-centred source matrices.
 template <int MAXS, int TW>
-__device__ __forceinline__ void tile_prologue(const ViewDesc &V, const Tile &t, float *s_ref,
-                                              SrcLds *s_src)
+__device__ __forceinline__ void tile_prologue(const ViewDesc &V, const PMConst &c, const Tile &t,
+                                              float *s_ref, SrcLds *s_src)
 {
     using T = TileShape<TW>;
     const int W = V.W, H = V.H;
@@ -625,7 +708,8 @@ __device__ __forceinline__ void tile_prologue(const ViewDesc &V, const Tile &t, 
         s_ref[k] = (float)V.ref[gy * W + gx];
     }
     if ((int)threadIdx.x < V.n_src)
-        setup_source(V, threadIdx.x, t.x0 + T::W / 2, t.y0 + T::H / 2, s_src[threadIdx.x]);
+        setup_source(V, c, threadIdx.x, t.x0 + T::W / 2, t.y0 + T::H / 2, T::W / 2, T::H / 2,
+                     s_src[threadIdx.x]);
 }
 
 // One red-black half-step on one tile (all 256 threads of the workgroup
@@ -640,7 +724,7 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
     const int iter = V.step >> 1, color = V.step & 1, pass = V.pass;
     const int W = V.W, H = V.H, S = V.n_src;
     const int tid = threadIdx.x;
-    tile_prologue<MAXS, TW>(V, t, s_ref, s_src);
+    tile_prologue<MAXS, TW>(V, c, t, s_ref, s_src);
     __syncthreads();
 
     constexpr int kHalf = TW / 2;               // active pixels per tile row
@@ -710,6 +794,18 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
             hn0[7] = n[0]; hn1[7] = n[1]; hn2[7] = n[2];
         }
 
+        // every candidate within the depth and slope bounds of the tile's
+        // `inner` sources (|a| = |n0 / n2| <= amax checked without dividing,
+        // with a margin for the division's rounding)
+        bool hyp_ok = true;
+        {
+            const float amx = 0.999f * slope_bound(c), dlo = hyp_depth_lo(V), dhi = hyp_depth_hi(V);
+#pragma unroll
+            for (int k = 0; k < kNHyp; ++k)
+                hyp_ok = hyp_ok && hd[k] >= dlo && hd[k] <= dhi && hn2[k] > 0.f &&
+                         fabsf(hn0[k]) <= amx * hn2[k] && fabsf(hn1[k]) <= amx * hn2[k];
+        }
+
         // ---- view selection on the current hypothesis --------------------
         const bool wtab = c.ntaps * c.ntaps <= kWTab;
         const RefStats R = ref_stats(c, s_ref, lbase, T::LW, wtab ? s_w + tid : nullptr);
@@ -728,16 +824,16 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
         for (int s0 = 0; s0 < S;) {
             const int left = S - s0;
             if (kGroupV >= 4 && (left >= 7 || left == 4 || left == 5)) {
-                select_group<4>(c, s_src, s0, du, dv, hd[0], a0, b0, s_ref, lbase, R, s_c0, tid);
+                select_group<4>(c, s_src, s0, du, dv, hd[0], a0, b0, s_ref, lbase, R, s_c0, tid, hyp_ok);
                 s0 += 4;
             } else if (kGroupV >= 3 && (left == 3 || left == 6)) {
-                select_group<3>(c, s_src, s0, du, dv, hd[0], a0, b0, s_ref, lbase, R, s_c0, tid);
+                select_group<3>(c, s_src, s0, du, dv, hd[0], a0, b0, s_ref, lbase, R, s_c0, tid, hyp_ok);
                 s0 += 3;
             } else if (kGroupV >= 2 && left >= 2) {
-                select_group<2>(c, s_src, s0, du, dv, hd[0], a0, b0, s_ref, lbase, R, s_c0, tid);
+                select_group<2>(c, s_src, s0, du, dv, hd[0], a0, b0, s_ref, lbase, R, s_c0, tid, hyp_ok);
                 s0 += 2;
             } else {
-                select_group<1>(c, s_src, s0, du, dv, hd[0], a0, b0, s_ref, lbase, R, s_c0, tid);
+                select_group<1>(c, s_src, s0, du, dv, hd[0], a0, b0, s_ref, lbase, R, s_c0, tid, hyp_ok);
                 s0 += 1;
             }
         }
@@ -885,7 +981,7 @@ __device__ __forceinline__ void halfstep_tile(const ViewDesc &V, const Tile &t, 
             if (e == 0u) continue;
             const SrcLds &Ss = s_src[s];
             const float cnt = (float)e;
-            eval_hyps<kGroupH, 0>(c, Ss, cnt, pass, du, dv, hd, ak, bk, s_ref, lbase, R, acc);
+            eval_hyps<kGroupH, 0>(c, Ss, cnt, pass, du, dv, hd, ak, bk, s_ref, lbase, R, acc, hyp_ok);
         }
         my_ncc += (kNHyp - 1) * nlist;
         if (pass == 1) my_geo += (kNHyp - 1) * nlist;
@@ -998,7 +1094,7 @@ __global__ __launch_bounds__(kThreads) void k_eval_cost(const ViewDesc *__restri
     __shared__ SrcLds s_src[MAXS];
     const Tile t = tiles[blockIdx.x];
     const ViewDesc &V = views[t.view];
-    tile_prologue<MAXS, kTileW>(V, t, s_ref, s_src);
+    tile_prologue<MAXS, kTileW>(V, c, t, s_ref, s_src);
     __syncthreads();
     const size_t HW = (size_t)V.W * V.H;
     for (int k = threadIdx.x; k < kTileW * kTileH; k += kThreads) {
@@ -1058,7 +1154,7 @@ __global__ __launch_bounds__(kThreads) void k_eval_stage(const ViewDesc *__restr
     __shared__ int s_box[4];
     const Tile t = tiles[blockIdx.x];
     const ViewDesc &V = views[t.view];
-    tile_prologue<MAXS, kTileW>(V, t, s_ref, s_src);
+    tile_prologue<MAXS, kTileW>(V, c, t, s_ref, s_src);
     __syncthreads();
     const size_t HW = (size_t)V.W * V.H;
     constexpr int kPix = kTileW * kTileH / kThreads;          // pixels per thread
@@ -1200,7 +1296,7 @@ __global__ __launch_bounds__(kThreads) void k_filter(const ViewDesc *__restrict_
     const Tile t = tiles[blockIdx.x];
     const ViewDesc &V = views[t.view];
     const bool wtab = c.ntaps * c.ntaps <= kWTab;
-    tile_prologue<MAXS, kTileW>(V, t, s_ref, s_src);
+    tile_prologue<MAXS, kTileW>(V, c, t, s_ref, s_src);
     __syncthreads();
     for (int k = threadIdx.x; k < kTileW * kTileH; k += kThreads) {
         const int tx = k & (kTileW - 1), ty = k / kTileW;

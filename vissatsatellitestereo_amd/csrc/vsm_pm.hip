@@ -418,6 +418,79 @@ __device__ __forceinline__ void window_pass(const PMConst &c, const SrcView (&SV
     if (t < T) consume(va, xa, ya, ra, wa);
 }
 
+// window_pass for the 6 x 6-tap window (radius 5, step 2) with tap weights in
+// LDS: the taps of a row are unrolled, so tap offsets, LDS offsets and the
+// register set of each tap are constants; the rows run in a loop.  Same
+// arithmetic in the same order as window_pass, so the same sums.
+template <int G, bool CLAMP>
+__device__ __forceinline__ void window_pass_r5s2(const PMConst &c, const SrcView (&SV)[G],
+                                                 const Proj (&P)[G], const float (&iz0)[G],
+                                                 const float (&sc)[G], const float *s_ref, int lbase,
+                                                 const RefStats &R, float (&ss)[G], float (&sss)[G],
+                                                 float (&srs)[G])
+{
+    constexpr int NT = 6, ST = 2, RAD = 5;
+    float giu[G], giv[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        giu[k] = -iz0[k] * iz0[k] * P[k].huz;
+        giv[k] = -iz0[k] * iz0[k] * P[k].hvz;
+        ss[k] = 0.f; sss[k] = 0.f; srs[k] = 0.f;
+    }
+    float rx[G], ry[G], rz[G];
+    row_start<G, false>(P, iz0, giu, giv, (float)(-RAD), -RAD, rx, ry, rz);
+    uint32_t v[2][G];
+    float fx[2][G], fy[2][G], rr[2], ww[2];
+    const float *ref_row = s_ref + lbase - RAD * R.ldsw - RAD;    // row -RAD, column -RAD
+    const float *wt_row = R.wt;
+    // gathers and LDS reads of tap ii of the current row into set ii & 1,
+    // after the accumulation that last used that set (tok)
+    auto issue = [&](int ii, const float (&tok)[G]) {
+        const int b = ii & 1;
+        float fi = (float)(ii * ST);
+#pragma unroll
+        for (int k = 0; k < G; ++k) asm volatile("" : "+v"(fi) : "v"(tok[k]));
+        rr[b] = ref_row[ii * ST];
+        ww[b] = wt_row[ii * kThreads];
+        __builtin_amdgcn_s_setprio(1);
+#pragma unroll
+        for (int k = 0; k < G; ++k)
+            tap_issue<false, CLAMP>(SV[k], P[k], giu[k], rx[k], ry[k], rz[k], fi, v[b][k], fx[b][k], fy[b][k]);
+        __builtin_amdgcn_s_setprio(0);
+    };
+    auto consume = [&](int ii) {
+        const int b = ii & 1;
+        const float r = rr[b] - R.rc;
+        const float w = ww[b];
+        const float wr = w * r;
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const float sv = blend_quad(v[b][k], fx[b][k], fy[b][k]) - sc[k];
+            const float ws = w * sv;
+            ss[k] += ws;
+            sss[k] = fmaf(ws, sv, sss[k]);
+            srs[k] = fmaf(wr, sv, srs[k]);
+        }
+    };
+    issue(0, ss);
+#pragma unroll 1
+    for (int jj = 0; jj < NT; ++jj) {
+#pragma unroll
+        for (int ii = 0; ii < NT; ++ii) {
+            if (ii + 1 < NT) {
+                issue(ii + 1, ss);
+            } else if (jj + 1 < NT) {
+                // first tap of the next row into set 0 (free: tap 4 was accumulated)
+                row_start<G, false>(P, iz0, giu, giv, (float)(-RAD), -RAD + (jj + 1) * ST, rx, ry, rz);
+                ref_row += ST * R.ldsw;
+                wt_row += NT * kThreads;
+                issue(0, ss);
+            }
+            consume(ii);
+        }
+    }
+}
+
 // NCC cost of G plane hypotheses.  Each hypothesis' source samples are
 // centred on the source texel at its window centre, the reference samples on
 // the reference centre value (NCC is invariant to the offsets; they keep the
@@ -487,6 +560,11 @@ __device__ __forceinline__ void ncc_cost_group(const PMConst &c, const SrcLds *c
             window_pass<1, false, false, false, true>(c, LV, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
     } else if (__ballot(far) != 0ull) {
         window_pass<G, true, true, false>(c, SV, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
+    } else if (R.wt != nullptr && c.radius == 5 && c.step == 2) {
+        if (__ballot(edge) != 0ull)
+            window_pass_r5s2<G, true>(c, SV, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
+        else
+            window_pass_r5s2<G, false>(c, SV, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);
     } else if (R.wt != nullptr) {
         if (__ballot(edge) != 0ull)
             window_pass<G, false, true, true>(c, SV, P, iz0, sc, s_ref, lbase, R, ss, sss, srs);

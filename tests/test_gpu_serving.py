@@ -128,10 +128,13 @@ def test_retire_failure_affects_only_its_request(tiny):
     """A request whose retirement fails (its filter's tile list cannot be
     allocated once the budget is cut below what is in use) ends with that
     error; a request retiring in the same step without the filter, and the
-    serve steps after it, are unaffected."""
+    serve steps after it, are unaffected.  With 16-wide half-step tiles the
+    filter (32-wide tiles) builds its own tile list at retirement, which is
+    the allocation that fails here; with 32-wide tiles it reuses the
+    request's and allocates nothing."""
     from vissatsatellitestereo_amd._lib import VsmError
     views = _views(tiny)
-    ctx = _ctx(tiny, num_iterations=2)
+    ctx = _ctx(tiny, num_iterations=2, tile_width=16)
     try:
         alone = {}
         for v in views[:2]:

@@ -76,6 +76,7 @@ struct ServeReq {
     int host_step = 0;
     int slot = -1;                          // its descriptor in the served view array
     Tile *dtiles = nullptr;                 // its half-step tiles (view index = slot)
+    std::vector<Tile> htiles;               // their host copy (source of the upload)
     int ntiles = 0, tile_w = 0;
     // results (device) when done: depth, normal (3), cost, consistency mask
     float *out_depth = nullptr, *out_normal = nullptr, *out_cost = nullptr;
@@ -1752,6 +1753,8 @@ static void serve_drop_keep(vsm_ctx *c, ServeReq &r)
 // everything it allocated is released again.
 // The request's half-step tiles at the context's tile width on the device,
 // its slot as their view index.
+// The host copy stays with the request until its tiles are rebuilt or it
+// retires, so the asynchronous upload needs no wait here.
 static int serve_build_tiles(vsm_ctx *c, ServeReq &r)
 {
     const int tw = c->params.tile_width;
@@ -1760,15 +1763,20 @@ static int serve_build_tiles(vsm_ctx *c, ServeReq &r)
     Tile *dt = nullptr;
     int rc = dalloc(c, &dt, tl.size());
     if (rc) return rc;
-    hipError_t e = hipMemcpyAsync(dt, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    const hipError_t e = hipMemcpyAsync(dt, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice,
+                                        c->stream);
     if (e != hipSuccess) {
         dfree(c, dt);
         return fail(c, VSM_EDEVICE, "request tiles: %s", hipGetErrorString(e));
     }
-    dfree(c, r.dtiles);
+    if (r.dtiles) {
+        // the previous tiles may still be read by queued work
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        dfree(c, r.dtiles);
+    }
     r.dtiles = dt;
-    r.ntiles = (int)tl.size();
+    r.htiles.swap(tl);
+    r.ntiles = (int)r.htiles.size();
     r.tile_w = tw;
     c->serve_dirty = true;
     return VSM_OK;
@@ -1781,6 +1789,7 @@ static void serve_release_slot(vsm_ctx *c, ServeReq &r)
     dfree(c, r.dtiles);
     r.dtiles = nullptr;
     r.ntiles = 0;
+    std::vector<Tile>().swap(r.htiles);
     c->serve_dirty = true;
 }
 
@@ -1897,7 +1906,11 @@ static int serve_finish(vsm_ctx *c, ServeReq &r, ViewDesc *dv)
     const PMConst pc = make_const(c->params);
     r.vs.desc.step = r.vs.desc.num_steps;
     HIPCHK(c, hipMemcpyAsync(dv, &r.vs.desc, sizeof(ViewDesc), hipMemcpyHostToDevice, c->stream));
-    if (r.filter) {
+    if (r.filter && r.tile_w == kTileW && r.dtiles) {
+        // the request's own half-step tiles have the filter's shape
+        HIPCHK(c, launch_filter(r.vs.desc.n_src, c->serve_views, r.dtiles, r.ntiles, pc, r.out_mask,
+                                c->stream));
+    } else if (r.filter) {
         std::vector<Tile> tl;
         tiles_for(r.vs.desc, 0, tl);
         Tile *dt = nullptr;

@@ -671,7 +671,9 @@ def bench_serving(a):
     for v in sites[0]["views"][:2]:
         warm.submit(v)
     warm.drain()
-    loop = ServeLoop(ctx, on_complete=on_complete, geometric=True)
+    # results stay on the device (the per-site fusion reads them there), as
+    # in the one-at-a-time queue below: neither form copies maps to the host
+    loop = ServeLoop(ctx, on_complete=on_complete, geometric=True, want=())
     ctx.reset_stats()
     ctx.set_timing(True)
     served = loop.run_schedule(arrivals, reqs)
@@ -692,7 +694,7 @@ def bench_serving(a):
     # capacity: every request queued at t = 0 (at the configured rate the
     # served form keeps up with the arrivals and its views/s is the offered load)
     ctx.set_params(window_step=step, tile_width=tile_w)
-    burst = ServeLoop(ctx, geometric=True).run_schedule([0.0] * len(reqs), reqs)
+    burst = ServeLoop(ctx, geometric=True, want=()).run_schedule([0.0] * len(reqs), reqs)
     ctx.set_params(window_step=1, tile_width=32)
     plain_cap = queue_schedule(ctx, [0.0] * len(reqs), reqs)
     log(f"capacity: served {burst['views_per_s']:.2f} views/s, plain {plain_cap['views_per_s']:.2f} views/s")

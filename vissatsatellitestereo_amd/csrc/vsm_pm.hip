@@ -220,12 +220,16 @@ __device__ __forceinline__ SrcView view_of(const SrcLds &S)
     return v;
 }
 
+// Source samples are blended in units of 2^-24 grey levels (blend_quad).
+constexpr float kQuadScale = 0x1p-24f;
+
 // Image value of the texel containing anchor-relative (px, py) (inside the
-// image): the offset the source samples of a window are centred on.
+// image), times kQuadScale: the offset the source samples of a window are
+// centred on.
 __device__ __forceinline__ float texel_at(const SrcView &S, float px, float py)
 {
     const int idx = __mul24(cvt_floor(py), S.W) + (cvt_floor(px) + S.base);
-    return (float)(((gquad_t)S.quad)[(unsigned)idx] & 0xFFu);
+    return (float)(((gquad_t)S.quad)[(unsigned)idx] & 0xFFu) * kQuadScale;
 }
 
 // Bilateral weight of a tap: 2^-(r^2 ccc + (dx^2+dy^2) css), the hardware
@@ -303,12 +307,26 @@ __device__ __forceinline__ void tap_issue(const SrcView &S, const Proj &P, float
     }
 }
 
+// Bilinear value of the quad texel (a, b, c, d), times kQuadScale:
+// top = fma(fx, b - a, a), bot = fma(fx, d - c, c), fma(fy, bot - top, top).
+// The bytes go into the low bits of binary16 lanes, where they read as the
+// subnormals a 2^-24 .. d 2^-24 (exact); one packed binary16 subtraction
+// forms (b - a, d - c) 2^-24 (exact), and the row FMAs take their binary16
+// operands as mixed-precision sources.  Each FP32 operation sees the
+// unscaled blend's operands times 2^-24 and no result falls below the
+// smallest normal float, so the value is the unscaled blend's times 2^-24
+// exactly (2 to 3 fewer VALU instructions per tap than converting the four
+// bytes).  The window's source moment sums then carry kQuadScale and
+// kQuadScale^2, which the NCC cancels.
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
 __device__ __forceinline__ float blend_quad(uint32_t v, float fx, float fy)
 {
-    const float a = (float)(v & 0xFFu), b = (float)((v >> 8) & 0xFFu);
-    const float c = (float)((v >> 16) & 0xFFu), d = (float)(v >> 24);
-    const float top = fmaf(fx, b - a, a);
-    const float bot = fmaf(fx, d - c, c);
+    const half2_t ac = __builtin_bit_cast(half2_t, v & 0x00FF00FFu);               // a | c << 16
+    const half2_t bd = __builtin_bit_cast(half2_t, __builtin_amdgcn_perm(0u, v, 0x0c030c01u)); // b | d << 16
+    const half2_t df = bd - ac;
+    const float top = fmaf(fx, (float)df.x, (float)ac.x);
+    const float bot = fmaf(fx, (float)df.y, (float)ac.y);
     return fmaf(fy, bot - top, top);
 }
 
@@ -581,7 +599,8 @@ __device__ __forceinline__ void ncc_cost_group(const PMConst &c, const SrcLds *c
         const float ms = ss[k] * R.inv_sw;
         const float var_s = sss[k] * R.inv_sw - ms * ms;
         const float cov = srs[k] * R.inv_sw - R.mr * ms;
-        const bool flat = R.var_r < c.min_var || var_s < c.min_var;
+        // var_s is in units of kQuadScale^2 (NCC itself is scale-free)
+        const bool flat = R.var_r < c.min_var || var_s < c.min_var * (kQuadScale * kQuadScale);
         const float ncc = cov * __builtin_amdgcn_rsqf(R.var_r * var_s);
         cost[k] = (!valid[k] || flat) ? kMaxCost : fminf(fmaxf(1.f - ncc, 0.f), kMaxCost);
     }

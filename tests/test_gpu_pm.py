@@ -548,7 +548,8 @@ def test_api_rejects_bad_input_and_stays_usable(tiny):
     finally:
         fresh.close()
     assert np.array_equal(got, ref)
-    small = MvsContext(device=0, hbm_budget_bytes=1 << 16)
+    # a 160 x 128 image holds 20 KB of pixels and a 40 KB pair texture
+    small = MvsContext(device=0, hbm_budget_bytes=1 << 15)
     try:
         with pytest.raises(VsmError):
             small.add_image(1, tiny["imgs"][names[0]], tiny["P4"][names[0]])

@@ -322,11 +322,15 @@ int main()
                     if (ax < 0 || ax >= W || ay < 0 || ay >= H) { ++bad; continue; }
                     // every tap clamped to the image box, relative to the anchor
                     const int xs[] = {-ax, 0, W - 1 - ax}, ys[] = {-ay, 0, H - 1 - ay};
-                    const int W4 = 4 * W, base4 = (int)(uint32_t)(4u * (uint32_t)(ay * W + ax));
+                    // pair texture: W + 1 texels of 2 bytes per row
+                    const int P = W + 1;
+                    const int Pb = 2 * P, baseb = (int)(uint32_t)(2u * (uint32_t)(ay * P + ax));
                     for (int ix : xs)
                         for (int iy : ys) {
-                            const uint32_t off = (uint32_t)(mul24(iy, W4) + ((ix << 2) + base4));
-                            const int64_t want = 4ll * ((int64_t)(ay + iy) * W + ax + ix);
+                            const uint32_t off = (uint32_t)(mul24(iy, Pb) + ((ix << 1) + baseb));
+                            const int64_t want = 2ll * ((int64_t)(ay + iy) * P + ax + ix);
+                            // the 4-byte word at the offset stays inside the texture
+                            if (want + 4 > 2ll * P * H) ++bad;
                             ++n;
                             if ((int64_t)off != want) ++bad;
                         }
@@ -341,7 +345,8 @@ def test_source_anchor_keeps_gather_offsets_in_range(tmp_path):
     """The per-tile source anchor stays inside the image for projections far
     outside it, behind the camera, at infinity or NaN, so the 24-bit offset
     arithmetic of the window gathers gives the exact texel offset for every
-    tap clamped to the image box (images up to 32768 px a side)."""
+    tap clamped to the image box, and the tap's 4-byte word stays inside the
+    pair texture (2-byte texels, W + 1 per row; images up to 32768 px a side)."""
     import shutil
     import subprocess
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"

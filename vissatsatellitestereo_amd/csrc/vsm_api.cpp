@@ -32,7 +32,7 @@ namespace {
 struct ImageRec {
     int id = 0, W = 0, H = 0;
     uint8_t *gray = nullptr;
-    uint32_t *quad = nullptr;
+    uint16_t *pair = nullptr;               // pair texture, (W + 1) x H texels
     double P4[16];
     double Pinv[16];
     float *depth[2] = {nullptr, nullptr};   // photometric / geometric depth kept on device
@@ -365,7 +365,7 @@ int build_geometry(vsm_ctx *c, const ImageRec &ref, const std::vector<const Imag
             V.g[s][r] = (float)(J[3 * r] * v[0] + J[3 * r + 1] * v[1] + J[3 * r + 2] * v[2]);
         V.src_w[s] = S.W;
         V.src_h[s] = S.H;
-        V.src_quad[s] = S.quad;
+        V.src_pair[s] = (const uint8_t *)S.pair;
         V.src_depth4[s] = nullptr;
     }
     // Rotation of the reference camera from P[:3,:3] = K R.
@@ -488,7 +488,7 @@ int prepare_view(vsm_ctx *c, const vsm_view_req &r, int pass, uint32_t view_id, 
     auto it = c->images.find(r.ref_id);
     if (it == c->images.end()) return fail(c, VSM_ENOENT, "unknown reference image %d", r.ref_id);
     const ImageRec &ref = it->second;
-    if (!ref.quad) return fail(c, VSM_EINVAL, "image %d was registered without pixels", r.ref_id);
+    if (!ref.pair) return fail(c, VSM_EINVAL, "image %d was registered without pixels", r.ref_id);
     if (r.n_src < 1 || r.n_src > VSM_MAX_SRC)
         return fail(c, VSM_EINVAL, "n_src %d outside [1, %d]", r.n_src, VSM_MAX_SRC);
     if (!r.src_ids) return fail(c, VSM_EINVAL, "src_ids is NULL");
@@ -497,7 +497,7 @@ int prepare_view(vsm_ctx *c, const vsm_view_req &r, int pass, uint32_t view_id, 
         auto si = c->images.find(r.src_ids[s]);
         if (si == c->images.end()) return fail(c, VSM_ENOENT, "unknown source image %d", r.src_ids[s]);
         if (r.src_ids[s] == r.ref_id) return fail(c, VSM_EINVAL, "source %d is the reference", r.ref_id);
-        if (!si->second.quad) return fail(c, VSM_EINVAL, "image %d was registered without pixels", r.src_ids[s]);
+        if (!si->second.pair) return fail(c, VSM_EINVAL, "image %d was registered without pixels", r.src_ids[s]);
         srcs.push_back(&si->second);
     }
     float dmin = c->params.depth_min > 0 ? c->params.depth_min : r.depth_min;
@@ -994,10 +994,10 @@ int vsm_add_image(vsm_ctx *c, int32_t id, const uint8_t *gray, int32_t w, int32_
     if (!inv4(P4, r.Pinv)) return fail(c, VSM_EINVAL, "P4 of image %d is singular", id);
     if (gray) {
         int rc = dalloc(c, &r.gray, (size_t)w * h);
-        if (!rc) rc = dalloc(c, &r.quad, (size_t)w * h);
+        if (!rc) rc = dalloc(c, &r.pair, (size_t)(w + 1) * h);
         if (rc) { dfree(c, r.gray); return rc; }
         HIPCHK(c, hipMemcpy2DAsync(r.gray, w, gray, stride, w, h, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, launch_pack_quad(r.gray, w, h, r.quad, c->stream));
+        HIPCHK(c, launch_pack_pair(r.gray, w, h, r.pair, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     c->images[id] = r;
@@ -1025,7 +1025,7 @@ int vsm_remove_image(vsm_ctx *c, int32_t id)
     (void)hipStreamSynchronize(c->stream);
     ImageRec &r = it->second;
     dfree(c, r.gray);
-    dfree(c, r.quad);
+    dfree(c, r.pair);
     dfree(c, r.depth[0]);
     dfree(c, r.depth[1]);
     dfree(c, r.normal[0]);

@@ -51,7 +51,7 @@ struct ViewDesc {
     float J[9];                     // d(x,y,d)/dX at the scene centre, normalised
     float N_out[9];                 // (col,row,d)-frame normal -> camera frame
     float N_in[9];                  // camera frame -> (col,row,d) frame
-    const uint32_t *src_quad[kMaxSrc];
+    const uint8_t *src_pair[kMaxSrc];   // pair texture of each source ((W + 1) x H x 2 B)
     const float4 *src_depth4[kMaxSrc];   // photometric depth of each source, 2x2 per texel
     int src_w[kMaxSrc], src_h[kMaxSrc];
     float ptri[kMaxSrc];            // triangulation prior

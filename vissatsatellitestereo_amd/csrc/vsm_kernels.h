@@ -31,7 +31,7 @@ hipError_t launch_load_hyp(const ViewDesc *views, int view, int npix, const floa
                            const float *normal, float cos_max_normal, hipStream_t st);
 hipError_t launch_advance(ViewDesc *views, int nviews, hipStream_t st);
 hipError_t launch_pack_depth4(const float *depth, int W, int H, float4 *out, hipStream_t st);
-hipError_t launch_pack_quad(const uint8_t *gray, int W, int H, uint32_t *quad, hipStream_t st);
+hipError_t launch_pack_pair(const uint8_t *gray, int W, int H, uint16_t *pair, hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long *p, int n, unsigned long long v, hipStream_t st);
 
 // Fusion (vsm_fuse.hip)

@@ -58,9 +58,10 @@ struct SrcLds {
     float A[12];                 // tile-centred ref -> src (anchor-relative), normalised
     float B[12];                 // src (anchor-relative) -> ref (tile-centre-relative)
     int ax, ay, W, H;
-    int base;                    // ay * W + ax: texel offset of the anchor
+    int pitch;                   // W + 1: texels per row of the pair texture
+    int base;                    // ay * pitch + ax: texel offset of the anchor
     float lox, hix, loy, hiy;    // in-image box in anchor-relative coordinates
-    const uint32_t *quad;
+    const uint8_t *pair;         // pair texture (k_pack_pair)
     const float4 *depth4;        // photometric depth of the source, 2x2 per texel
     float ptri, tri, g0, g1, g2;
     int inner;                   // every window of the tile lies inside the source (see below)
@@ -149,10 +150,11 @@ __device__ void setup_source(const ViewDesc &V, const PMConst &c, int s, int uc,
     for (int k = 0; k < 12; ++k) { o.A[k] = (float)(A[k] * ma); o.B[k] = (float)(B[k] * mb); }
     o.ax = ax; o.ay = ay;
     o.W = V.src_w[s]; o.H = V.src_h[s];
-    o.base = ay * o.W + ax;
+    o.pitch = o.W + 1;
+    o.base = ay * o.pitch + ax;
     o.lox = (float)(-ax); o.hix = (float)(o.W - 1 - ax);
     o.loy = (float)(-ay); o.hiy = (float)(o.H - 1 - ay);
-    o.quad = V.src_quad[s];
+    o.pair = V.src_pair[s];
     o.depth4 = V.src_depth4[s];
     o.ptri = V.ptri[s]; o.tri = V.tri[s];
     o.g0 = V.g[s][0]; o.g1 = V.g[s][1]; o.g2 = V.g[s][2];
@@ -181,9 +183,13 @@ __device__ __forceinline__ Proj make_proj(const SrcLds &S, float du, float dv, f
     return P;
 }
 
-typedef const __attribute__((address_space(1))) uint32_t *gquad_t;
+typedef const __attribute__((address_space(1))) uint8_t *gbyte_t;
 typedef const __attribute__((address_space(1))) float *gfloat_t;
 typedef const __attribute__((address_space(3))) uint32_t *lquad_t;
+// A 4-byte word at a 2-byte boundary of the pair texture: the HSA runtime
+// enables unaligned global access, so this is one global_load_dword.
+typedef uint32_t u32_any __attribute__((aligned(1)));
+typedef const __attribute__((address_space(1))) u32_any *gword_t;
 
 // floor(x) as an integer in one instruction (x finite).
 __device__ __forceinline__ int cvt_floor(float x)
@@ -197,26 +203,26 @@ __device__ __forceinline__ int cvt_floor(float x)
 // once per evaluation).
 struct SrcView {
     float lox, hix, loy, hiy;
-    int W, base;
-    int W4, base4;               // row pitch and anchor offset in bytes
-    const uint32_t *quad;
+    int pitch, base;             // in texels
+    int pitchb, baseb;           // row pitch and anchor offset in bytes
+    const uint8_t *pair;
 };
 
 // Every lane of a wave reads the same source in a window pass, so the texture
 // pointer, row pitch and anchor offset go to scalar registers and each gather
-// is a scalar base plus a 32-bit byte offset (4 * texel index < 2^32 for
-// images of at most 2^30 texels).
+// is a scalar base plus a 32-bit byte offset (2 * texel index < 2^32 for
+// images of at most 2^30 pixels).
 __device__ __forceinline__ SrcView view_of(const SrcLds &S)
 {
     SrcView v;
     v.lox = S.lox; v.hix = S.hix; v.loy = S.loy; v.hiy = S.hiy;
-    v.W = S.W; v.base = S.base;
-    v.W4 = __builtin_amdgcn_readfirstlane(S.W * 4);
-    v.base4 = __builtin_amdgcn_readfirstlane(S.base * 4);
-    const uint64_t a = (uint64_t)S.quad;
+    v.pitch = S.pitch; v.base = S.base;
+    v.pitchb = __builtin_amdgcn_readfirstlane(S.pitch * 2);
+    v.baseb = __builtin_amdgcn_readfirstlane(S.base * 2);
+    const uint64_t a = (uint64_t)S.pair;
     const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a);
     const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a >> 32));
-    v.quad = (const uint32_t *)(lo | (hi << 32));
+    v.pair = (const uint8_t *)(lo | (hi << 32));
     return v;
 }
 
@@ -228,8 +234,8 @@ constexpr float kQuadScale = 0x1p-24f;
 // centred on.
 __device__ __forceinline__ float texel_at(const SrcView &S, float px, float py)
 {
-    const int idx = __mul24(cvt_floor(py), S.W) + (cvt_floor(px) + S.base);
-    return (float)(((gquad_t)S.quad)[(unsigned)idx] & 0xFFu) * kQuadScale;
+    const int idx = __mul24(cvt_floor(py), S.pitch) + (cvt_floor(px) + S.base);
+    return (float)((gbyte_t)S.pair)[2u * (unsigned)idx] * kQuadScale;
 }
 
 // Bilateral weight of a tap: 2^-(r^2 ccc + (dx^2+dy^2) css), the hardware
@@ -280,13 +286,14 @@ __device__ __forceinline__ RefStats ref_stats(const PMConst &c, const float *s_r
 }
 
 // One bilinear tap split in two: tap_issue forms the anchor-relative source
-// position of column offset fi and starts the gather of its quad texel;
-// blend_quad interpolates the loaded texel (DN).  The texel index
-// iy*W + ix is formed with a signed 24-bit multiply-add on anchor-relative
-// integers plus the anchor's offset ay*W + ax (|relative coordinates| < 2^23,
-// image side <= 32768).  With LDSRC the texel comes from a box of the
-// source staged in LDS (S.W4 and S.base4 are then the box pitch and the
-// anchor's index in it, in texels).
+// position of column offset fi and starts the gather of its quad, the word
+// of texels (ix, iy) and (ix + 1, iy) of the pair texture; blend_quad
+// interpolates it (DN).  The byte offset 2 (iy*pitch + ix) is formed with a
+// signed 24-bit multiply-add on anchor-relative integers plus the anchor's
+// offset 2 (ay*pitch + ax) (|relative coordinates| < 2^23, image side <=
+// 32768).  With LDSRC the quad comes from a box of the source staged in LDS
+// as aligned words (S.pitchb and S.baseb are then the box pitch and the
+// anchor's index in it, in words).
 template <bool EXACT, bool CLAMP, bool LDSRC = false>
 __device__ __forceinline__ void tap_issue(const SrcView &S, const Proj &P, float giu, float rx, float ry,
                                           float rz, float fi, uint32_t &v, float &fx, float &fy)
@@ -300,14 +307,15 @@ __device__ __forceinline__ void tap_issue(const SrcView &S, const Proj &P, float
     fx = __builtin_amdgcn_fractf(px);
     fy = __builtin_amdgcn_fractf(py);
     if constexpr (LDSRC) {
-        v = ((lquad_t)S.quad)[__mul24(iy, S.W4) + ix + S.base4];
+        v = ((lquad_t)S.pair)[__mul24(iy, S.pitchb) + ix + S.baseb];
     } else {
-        const unsigned off = (unsigned)(__mul24(iy, S.W4) + ((ix << 2) + S.base4));
-        v = *(gquad_t)((const __attribute__((address_space(1))) char *)S.quad + off);
+        const unsigned off = (unsigned)(__mul24(iy, S.pitchb) + ((ix << 1) + S.baseb));
+        v = *(gword_t)((gbyte_t)S.pair + off);
     }
 }
 
-// Bilinear value of the quad texel (a, b, c, d), times kQuadScale:
+// Bilinear value of the quad (bytes a = I(x,y), c = I(x,y+1), b = I(x+1,y),
+// d = I(x+1,y+1)), times kQuadScale:
 // top = fma(fx, b - a, a), bot = fma(fx, d - c, c), fma(fy, bot - top, top).
 // The bytes go into the low bits of binary16 lanes, where they read as the
 // subnormals a 2^-24 .. d 2^-24 (exact); one packed binary16 subtraction
@@ -322,8 +330,8 @@ typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ float blend_quad(uint32_t v, float fx, float fy)
 {
-    const half2_t ac = __builtin_bit_cast(half2_t, v & 0x00FF00FFu);               // a | c << 16
-    const half2_t bd = __builtin_bit_cast(half2_t, __builtin_amdgcn_perm(0u, v, 0x0c030c01u)); // b | d << 16
+    const half2_t ac = __builtin_bit_cast(half2_t, __builtin_amdgcn_perm(0u, v, 0x0c010c00u)); // a | c << 16
+    const half2_t bd = __builtin_bit_cast(half2_t, __builtin_amdgcn_perm(0u, v, 0x0c030c02u)); // b | d << 16
     const half2_t df = bd - ac;
     const float top = fmaf(fx, (float)df.x, (float)ac.x);
     const float bot = fmaf(fx, (float)df.y, (float)ac.y);
@@ -1219,9 +1227,9 @@ __global__ __launch_bounds__(kThreads) void k_eval_cost(const ViewDesc *__restri
 // workgroup reduces the bounding box of every window's projection (the
 // projective image of a square window is the convex hull of its corners'
 // images; +1 texel for the bilinear neighbour, clamped to the image), and
-// when the box has at most kStageCap texels it copies that box of the quad
-// texture into LDS with row-contiguous loads, after which every tap reads
-// LDS instead of the L1/L2-cached global texture.  STAGE = false runs the
+// when the box has at most kStageCap texels it copies the box's quads (words
+// of the pair texture) into LDS with row-contiguous loads, after which every
+// tap reads LDS instead of the L1/L2-cached global texture.  STAGE = false runs the
 // same loops with global gathers only, so the two variants time the gather
 // path alone; their outputs are identical (same texels, same arithmetic).
 // ---------------------------------------------------------------------------
@@ -1320,17 +1328,16 @@ __global__ __launch_bounds__(kThreads) void k_eval_stage(const ViewDesc *__restr
             if (staged) {
                 // row-contiguous copy of the box
                 int xx = threadIdx.x % bw, yy = threadIdx.x / bw;
-                const uint32_t *src = Ss.quad;
                 for (int i = threadIdx.x; i < bw * bh; i += kThreads) {
-                    s_stage[i] = src[(size_t)(by0 + yy) * Ss.W + bx0 + xx];
+                    s_stage[i] = *(gword_t)((gbyte_t)Ss.pair + 2 * ((size_t)(by0 + yy) * Ss.pitch + bx0 + xx));
                     xx += kThreads % bw;
                     yy += kThreads / bw;
                     if (xx >= bw) { xx -= bw; ++yy; }
                 }
                 lv = view_of(Ss);
-                lv.W4 = bw;
-                lv.base4 = (Ss.ay - by0) * bw + (Ss.ax - bx0);
-                lv.quad = s_stage;
+                lv.pitchb = bw;
+                lv.baseb = (Ss.ay - by0) * bw + (Ss.ax - bx0);
+                lv.pair = (const uint8_t *)s_stage;
                 if (threadIdx.x == 0 && n_staged) atomicAdd(n_staged, 1u);
             }
             __syncthreads();
@@ -1496,18 +1503,19 @@ __global__ void k_advance(ViewDesc *views, int nviews)
     if (v < nviews && views[v].active && views[v].step < views[v].num_steps) views[v].step += 1;
 }
 
-// Quad-packed texture: texel (x,y) holds I(x,y), I(x1,y), I(x,y1), I(x1,y1)
-// with x1 = min(x+1, W-1), y1 = min(y+1, H-1) -- one dword per bilinear tap.
-__global__ void k_pack_quad(const uint8_t *__restrict__ gray, int W, int H,
-                            uint32_t *__restrict__ quad)
+// Pair texture: W + 1 texels of 2 bytes per row; texel (x, y) holds I(x', y)
+// and I(x', y1) with x' = min(x, W-1), y1 = min(y+1, H-1).  The word at texel
+// x < W is the bilinear quad of (x, y) with edge-clamped neighbours (bytes
+// I(x,y), I(x,y1), I(x1,y), I(x1,y1), x1 = min(x+1, W-1)): one 4-byte gather
+// per tap from 2 bytes per pixel.
+__global__ void k_pack_pair(const uint8_t *__restrict__ gray, int W, int H, uint16_t *__restrict__ pair)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
-    if (x >= W || y >= H) return;
-    const int x1 = min(x + 1, W - 1), y1 = min(y + 1, H - 1);
-    const uint32_t a = gray[y * W + x], b = gray[y * W + x1];
-    const uint32_t c = gray[y1 * W + x], d = gray[y1 * W + x1];
-    quad[y * W + x] = a | (b << 8) | (c << 16) | (d << 24);
+    if (x > W || y >= H) return;
+    const int xs = min(x, W - 1), y1 = min(y + 1, H - 1);
+    const uint32_t a = gray[(size_t)y * W + xs], c = gray[(size_t)y1 * W + xs];
+    pair[(size_t)y * (W + 1) + x] = (uint16_t)(a | (c << 8));
 }
 
 // Depth texture for the geometric term: texel (x,y) holds d(x,y), d(x1,y),
@@ -1659,10 +1667,10 @@ hipError_t launch_pack_depth4(const float *depth, int W, int H, float4 *out, hip
     return hipGetLastError();
 }
 
-hipError_t launch_pack_quad(const uint8_t *gray, int W, int H, uint32_t *quad, hipStream_t st)
+hipError_t launch_pack_pair(const uint8_t *gray, int W, int H, uint16_t *pair, hipStream_t st)
 {
-    dim3 grid((W + 255) / 256, H);
-    k_pack_quad<<<grid, 256, 0, st>>>(gray, W, H, quad);
+    dim3 grid((W + 1 + 255) / 256, H);
+    k_pack_pair<<<grid, 256, 0, st>>>(gray, W, H, pair);
     return hipGetLastError();
 }
 

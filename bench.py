@@ -286,13 +286,16 @@ def roofline(st, cfg):
         gbps = traffic / (avg_ms * 1e-3) / 1e9
         profiled = {"hbm_gbps": round(gbps, 1), "hbm_frac": round(gbps / HBM_PEAK_GBPS, 4),
                     "valu_busy_pct": round(t["valubusy"], 1) if "valubusy" in t else None,
-                    "valu_lanes_active_pct": round(t["valuutil"], 1) if "valuutil" in t else None}
+                    "valu_lanes_active_pct": round(t["valuutil"], 1) if "valuutil" in t else None,
+                    "ta_busy_pct": round(t["ta_busy"], 1) if "ta_busy" in t else None,
+                    "td_busy_pct": round(t["td_busy"], 1) if "td_busy" in t else None}
     return {"bound": "valu", "kernel": "k_halfstep", "achieved": round(ach, 3),
             "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / FP32_PEAK_TFLOPS, 4),
             "traffic": traffic, "traffic_unit": "bytes per launch", "traffic_source": source,
             "profiled": profiled,
-            "bound_note": "vector FP32 (VALU): a gather/stencil kernel, neither MFMA nor HBM "
-                          "work; peak = MI355X FP32 157.3 TFLOP/s",
+            "bound_note": "priced against vector FP32 (VALU), MI355X 157.3 TFLOP/s: a gather/stencil "
+                          "kernel, neither MFMA nor HBM work; the profiled limiter is the texture "
+                          "data path of its gathers (td_busy_pct) with VALU issue close behind",
             "avg_launch_ms": round(avg_ms, 4), "taps_per_launch": int(taps),
             "flops_per_tap": FLOPS_PER_TAP, "timing": "HIP events on the context stream around "
                                                       "every k_halfstep launch of the timed steps"}

@@ -120,21 +120,28 @@ class ServeLoop:
         nxt = 0
         tickets = [None] * len(views)
         steps = 0
+        idle = 0       # consecutive steps that ended with nothing on the device
         while nxt < len(order) or self.pending:
             now = self.clock() - start
             while nxt < len(order) and arrivals[order[nxt]] <= now:
                 i = int(order[nxt])
                 tickets[i] = self.submit(views[i], arrival=start + arrivals[i])
                 nxt += 1
-            if self.pending:
+                idle = 0
+            if self.pending and idle < 2:
                 na, nq = self.step()
                 if nxt >= len(order):
                     self._check_progress(na, nq)
                 steps += 1
                 if steps > max_steps:
                     raise RuntimeError("serving did not drain")
+                # two idle steps in a row: what waits needs a request not yet
+                # arrived (the first may have retired what the queue waited on)
+                idle = idle + 1 if na == 0 else 0
             elif nxt < len(order):
                 time.sleep(min(idle_sleep, max(0.0, arrivals[order[nxt]] - now)))
+            else:
+                idle = 0
         wall = self.clock() - start
         lat = [self.done[t][2] - self.done[t][1] for t in tickets]
         return dict(tickets=tickets, latency_s=lat, wall_s=wall, steps=steps,

@@ -218,7 +218,7 @@ __device__ __forceinline__ SrcView view_of(const SrcLds &S)
     v.lox = S.lox; v.hix = S.hix; v.loy = S.loy; v.hiy = S.hiy;
     v.pitch = S.pitch; v.base = S.base;
     v.pitchb = __builtin_amdgcn_readfirstlane(S.pitch * 2);
-    v.baseb = __builtin_amdgcn_readfirstlane(S.base * 2);
+    v.baseb = __builtin_amdgcn_readfirstlane((int)(2u * (uint32_t)S.base));   // may exceed 2^31
     const uint64_t a = (uint64_t)S.pair;
     const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a);
     const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a >> 32));
@@ -309,7 +309,8 @@ __device__ __forceinline__ void tap_issue(const SrcView &S, const Proj &P, float
     if constexpr (LDSRC) {
         v = ((lquad_t)S.pair)[__mul24(iy, S.pitchb) + ix + S.baseb];
     } else {
-        const unsigned off = (unsigned)(__mul24(iy, S.pitchb) + ((ix << 1) + S.baseb));
+        // modulo-2^32 arithmetic; the sum is the byte offset, below 2^32
+        const unsigned off = (unsigned)__mul24(iy, S.pitchb) + (((unsigned)ix << 1) + (unsigned)S.baseb);
         v = *(gword_t)((gbyte_t)S.pair + off);
     }
 }

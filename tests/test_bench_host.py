@@ -110,3 +110,22 @@ def test_c2_oracle_fixture_and_bar():
     assert bar["dsm_coverage"] > 0.9 and bar["kept_fraction"] > 0.5
     assert bench.ORACLE_BAR_M == 0.05
     assert bench.oracle_bar("c9") is None
+
+
+def test_roofline_from_live_timing_and_committed_profile():
+    """roofline(): achieved = 30 FLOP x device-counted taps / live launch time;
+    traffic and the busy figures come from the committed counter profile of
+    the same workload (none for a workload without one)."""
+    import json
+    st = {"halfstep_ms": 2 * 1250.0, "halfstep_launches": 2, "taps": 2 * 1.104e12}
+    r = bench.roofline(st, "c5")
+    assert r["kernel"] == "k_halfstep" and r["unit"] == "TFLOP/s"
+    assert abs(r["achieved"] - 30 * 1.104e12 / 1.25 / 1e12) < 1e-3
+    assert abs(r["frac"] - r["achieved"] / r["peak"]) < 1e-4 and r["peak"] == 157.3
+    prof = json.load(open(os.path.join(ROOT, "profiles", "k_halfstep_c5_counters.json")))
+    assert r["traffic"] == round(prof["fetch_bytes"] + prof["write_bytes"])
+    p = r["profiled"]
+    assert p["td_busy_pct"] == round(prof["td_busy"], 1) and p["ta_busy_pct"] == round(prof["ta_busy"], 1)
+    assert abs(p["hbm_gbps"] - r["traffic"] / 1.25 / 1e9) < 0.1
+    assert bench.roofline(st, "c9")["traffic"] is None
+    assert bench.roofline({"halfstep_ms": 0.0, "halfstep_launches": 0, "taps": 0}, "c5") is None
